@@ -1,0 +1,136 @@
+/*
+ * temgym_hip.h -- C ABI of libtemgym_hip.so, the MI355X (gfx950) kernels behind
+ * temgymcore_amd's drop-in Ray / Component / run_to_end API.
+ *
+ * Conventions (every entry point):
+ *   - Plain pointers to device memory, int64 sizes, POD parameter structs.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *   - Work is enqueued on `stream`; no entry point synchronises the device
+ *     unless its comment says so.
+ *   - The caller owns and allocates every input, output and workspace buffer.
+ *   - Return value: 0 on success, a negative TG_E* code on failure.
+ *     tg_last_error() returns a thread-local description of the last failure.
+ *   - No C++ exception crosses this boundary.
+ */
+#ifndef TEMGYM_HIP_H
+#define TEMGYM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define TG_OK 0
+#define TG_EINVAL (-1)   /* argument validation failed            */
+#define TG_EHIP (-2)     /* a HIP runtime call failed              */
+#define TG_ENOSYS (-3)   /* requested variant not implemented      */
+
+/* Last error message of the calling thread ("" if none). */
+const char* tg_last_error(void);
+
+/* ABI version; bumped whenever a struct layout below changes. */
+int32_t tg_abi_version(void);  /* currently 2 */
+
+/* ======================================================================== *
+ * Ray branch: per-ray register programs
+ *
+ * Replaces the element-wise op stream that run_iter / run_to_end issue for a
+ * vectorised Ray (reference src/temgym_core/run.py:47-116, propagator.py:51-72,
+ * components.py:133-523) and the numba binning kernel Grid.into_image ->
+ * utils.inplace_sum (reference grid.py:231-280, utils.py:83-114).
+ *
+ * A program is a straight-line list of scalar instructions over a small
+ * per-ray register file.  Instruction immediates hold host-folded constants
+ * (parameters combined in double precision, then rounded once to the ray
+ * dtype, as weakly typed Python scalars are in the reference).
+ * ======================================================================== */
+
+#define TG_MAX_INSTR 96
+#define TG_MAX_REG 12
+#define TG_MAX_SLOT 10
+#define TG_MAX_DIM 4
+
+enum tg_opcode {
+  TG_OP_ADD = 0,    /* d = a + b                          */
+  TG_OP_SUB = 1,    /* d = a - b                          */
+  TG_OP_MUL = 2,    /* d = a * b                          */
+  TG_OP_DIV = 3,    /* d = a / b  (IEEE correctly rounded) */
+  TG_OP_ADDI = 4,   /* d = a + imm                        */
+  TG_OP_SUBI = 5,   /* d = a - imm                        */
+  TG_OP_RSUBI = 6,  /* d = imm - a                        */
+  TG_OP_MULI = 7,   /* d = a * imm                        */
+  TG_OP_DIVI = 8,   /* d = a / imm                        */
+  TG_OP_RDIVI = 9,  /* d = imm / a                        */
+  TG_OP_NEG = 10,   /* d = -a                             */
+  TG_OP_MOV = 11,   /* d = a                              */
+  TG_OP_SET = 12,   /* d = imm                            */
+  TG_OP_SIGN = 13,  /* d = sign(a): -1, 0, +1, NaN->NaN    */
+  TG_OP_COUNT = 14,
+  /* Flag or-ed into `op` in TG_F64 programs: the operation's operands and
+   * result are f32 values (JAX-style per-operation promotion of mixed f32/f64
+   * bundles).  The f64 kernel computes in f64 and rounds the result to f32,
+   * which is the correctly rounded f32 result for + - * / (53 >= 2*24+2). */
+  TG_OP_F32 = 0x80
+};
+
+enum tg_dtype { TG_F32 = 0, TG_F64 = 1 };
+
+typedef struct tg_instr {
+  uint8_t op, dst, a, b;
+  float immf;   /* imm rounded to f32 (used by TG_F32 programs) */
+  double imm;   /* imm in f64 (used by TG_F64 programs)         */
+} tg_instr;     /* 16 bytes */
+
+/* One varying operand: a device array broadcast over the logical shape.
+ * element offset of multi-index (i0..i3) = sum_k idx_k * stride[k]. */
+typedef struct tg_slot {
+  const void* ptr;
+  int64_t stride[TG_MAX_DIM];
+  int32_t reg;      /* register the value is loaded into */
+  int16_t contig;   /* 1: element i of the flattened shape is ptr[i] */
+  int16_t dtype;    /* tg_dtype of the array (f32 arrays may feed f64 programs) */
+} tg_slot;
+
+typedef struct tg_program {
+  int32_t dtype;             /* tg_dtype                                   */
+  int32_t n_instr;
+  int32_t n_slot;
+  int32_t n_out;
+  int64_t n;                 /* number of rays = prod(shape)               */
+  int64_t shape[TG_MAX_DIM]; /* logical broadcast shape, leading dims = 1  */
+  tg_instr instr[TG_MAX_INSTR];
+  tg_slot slot[TG_MAX_SLOT];
+  void* out[TG_MAX_SLOT];    /* contiguous outputs of n elements           */
+  int32_t out_reg[TG_MAX_SLOT];
+  int32_t out_dtype[TG_MAX_SLOT]; /* tg_dtype of each output array         */
+} tg_program;
+
+/* Detector binning fused after a program (or run alone with n_instr = 0).
+ * pixel (py, px) = round_half_even(M @ [y, x, 1]) computed in f64, then the
+ * reference's int32 cast (NaN -> 0, saturating) and 0 <= p < shape bounds. */
+typedef struct tg_bin_spec {
+  int32_t enabled;
+  int32_t acc_dtype;         /* 0 = int32, 1 = int64, 2 = float32, 3 = float64 */
+  int32_t x_reg, y_reg;      /* registers holding x and y (or -1: uniform)  */
+  double x_uniform, y_uniform;
+  double m[6];               /* metres->pixels rows: py = m0*y + m1*x + m2; px = m3*y + m4*x + m5 */
+  int64_t height, width;
+  void* acc;                 /* [height, width] accumulator, row-major      */
+  int32_t w_reg;             /* weight register (-1: weight 1)              */
+  int32_t pad;
+} tg_bin_spec;
+
+/* Run `prog` over prog->n rays; optionally bin the final positions.
+ * Ray-count 0 is a no-op. */
+int tg_ray_program_run(const tg_program* prog, const tg_bin_spec* bin, void* stream);
+
+/* Number of compute units of the current device (cached per device). */
+int32_t tg_device_cu_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TEMGYM_HIP_H */

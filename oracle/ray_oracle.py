@@ -1,0 +1,212 @@
+"""Ray-branch oracle (numpy) -- test infrastructure, see oracle/__init__.py.
+
+A ray is a dict with keys x, y, dx, dy, z, pathlength, one.  Values are numpy
+arrays (per ray) or Python floats (uniform).  numpy >= 2 follows NEP 50: a
+Python float meeting an f32 array is rounded to f32 and the operation runs in
+f32 -- the same "weak scalar" rule JAX applies to the reference, so writing
+the reference formulas with numpy arrays reproduces its rounding: every
+elementwise numpy operation is one IEEE operation, without FMA contraction.
+
+A model is a list of (kind, params) tuples:
+  ("plane", z) ("lens", z, f) ("detector", z) ("scanner", z, spx, spy, stx, sty)
+  ("descanner", z, spx, spy, stx, sty, err12) ("deflector", z, defx, defy)
+  ("rotator", z, angle_deg) ("biprism", z, def_x)
+
+Pinning (tests/test_oracle_ray.py):
+  * README golden: Ray(0.1, 0.2, 0.3, 0.4, 0, 0) through Lens(0.5, f=1) and
+    Detector(1.0) -> (0.275, 0.4, 0.05, 0.0, z=1.0, pathlength=0.88875)
+    (reference README.md:35-52), and its ABCD matrix (README.md:230-234);
+  * closed forms of the reference tests (test_rays.py:26-39,
+    test_component.py:186-259, :313-353);
+  * pixel-transform known-answer tables (test_component.py:73-174) and the
+    rotated scan-step vector (test_component.py:356-378);
+  * binning: brute-force per-ray loop == vectorised counts; sum == in-range
+    count; a planted off-by-one in the rounding fails the tables.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+FIELDS = ("x", "y", "dx", "dy", "z", "pathlength", "one")
+
+
+def make_ray(x, y, dx, dy, z=0.0, pathlength=0.0, one=1.0):
+    return dict(x=x, y=y, dx=dx, dy=dy, z=z, pathlength=pathlength, one=one)
+
+
+def propagate(r, d):
+    """FreeSpaceParaxial (reference propagator.py:51-72)."""
+    return dict(r, x=r["x"] + r["dx"] * d, y=r["y"] + r["dy"] * d, z=r["z"] + d,
+                pathlength=r["pathlength"] + d)
+
+
+def lens(r, f):
+    """Lens (reference components.py:161-174)."""
+    x, y = r["x"], r["y"]
+    return dict(r, dx=-x / f + r["dx"], dy=-y / f + r["dy"],
+                pathlength=r["pathlength"] - (x * x + y * y) / (2 * f), one=r["one"] * 1.0)
+
+
+def scanner(r, spx, spy, stx=0.0, sty=0.0):
+    """Scanner (reference components.py:279-285)."""
+    o = r["one"]
+    return dict(r, x=r["x"] + spx * o, y=r["y"] + spy * o, dx=r["dx"] + stx * o, dy=r["dy"] + sty * o)
+
+
+def descanner(r, spx, spy, stx, sty, e):
+    """Descanner with the 12 DescanError coefficients in NamedTuple order (reference components.py:320-372)."""
+    o = r["one"]
+    return dict(
+        r,
+        x=r["x"] + (spx * e[0] + spy * e[1] + e[8] - spx) * o,
+        y=r["y"] + (spx * e[2] + spy * e[3] + e[9] - spy) * o,
+        dx=r["dx"] + (spx * e[4] + spy * e[5] + e[10] - stx) * o,
+        dy=r["dy"] + (spx * e[6] + spy * e[7] + e[11] - sty) * o,
+    )
+
+
+def deflector(r, defx, defy):
+    """Deflector; pathlength uses incoming slopes (reference components.py:476-482)."""
+    x, y, dx, dy = r["x"], r["y"], r["dx"], r["dy"]
+    o = r["one"]
+    return dict(r, dx=dx + defx * o, dy=dy + defy * o, pathlength=r["pathlength"] + dx * x + dy * y)
+
+
+def rotator(r, angle_deg):
+    """Rotator (reference components.py:503-523); cos/sin of the angle in f64."""
+    a = angle_deg * (math.pi / 180)
+    c, s = math.cos(a), math.sin(a)
+    return dict(r, x=r["x"] * c - r["y"] * s, y=r["x"] * s + r["y"] * c,
+                dx=r["dx"] * c - r["dy"] * s, dy=r["dx"] * s + r["dy"] * c)
+
+
+def biprism(r, def_x):
+    """Biprism (reference components.py:553-559)."""
+    x, y, dx, dy = r["x"], r["y"], r["dx"], r["dy"]
+    return dict(r, dx=dx + def_x * r["one"] * np.sign(x), pathlength=r["pathlength"] + dx * x + dy * y)
+
+
+def apply(r, comp):
+    kind, *p = comp
+    if kind in ("plane", "detector"):
+        return r
+    if kind == "lens":
+        return lens(r, p[1])
+    if kind == "scanner":
+        return scanner(r, *p[1:])
+    if kind == "descanner":
+        return descanner(r, *p[1:])
+    if kind == "deflector":
+        return deflector(r, *p[1:])
+    if kind == "rotator":
+        return rotator(r, p[1])
+    if kind == "biprism":
+        return biprism(r, p[1])
+    raise ValueError(kind)
+
+
+def run_to_end(r, model):
+    """Step structure of reference run.py:75-82: propagate (even by 0) before every element."""
+    for comp in model:
+        r = propagate(r, comp[1] - r["z"])
+        r = apply(r, comp)
+    return r
+
+
+def run_steps(r, model):
+    """All intermediate states of run_iter (reference run.py:47-82)."""
+    out = []
+    for comp in model:
+        r = propagate(r, comp[1] - r["z"])
+        out.append(r)
+        r = apply(r, comp)
+        out.append(r)
+    return out
+
+
+# ---- pixel transforms (reference coordinate_transforms.py:7-140, grid.py:37-63) ----
+
+def pixels_to_metres_matrix(centre, pixel_size, shape, flip_y=False, rotation=0.0):
+    def rot(rad):
+        return np.array([[math.cos(rad), math.sin(rad), 0.0], [-math.sin(rad), math.cos(rad), 0.0], [0, 0, 1.0]])
+
+    def scale(p):
+        return np.array([[p[0], 0.0, 0.0], [0.0, p[1], 0.0], [0, 0, 1.0]])
+
+    def shift(c):
+        return np.array([[1.0, 0.0, c[0]], [0.0, 1.0, c[1]], [0, 0, 1.0]])
+
+    flip = np.array([[-1.0, 0, 0], [0, 1.0, 0], [0, 0, 1.0]])
+    cond = flip if flip_y else np.eye(3)
+    s = np.asarray(shape, dtype=np.float64)
+    return cond @ flip @ rot(math.pi / 180 * rotation) @ shift(centre) @ scale(pixel_size) @ shift(-(s - 1) / 2.0)
+
+
+def metres_to_pixels_matrix(centre, pixel_size, shape, flip_y=False, rotation=0.0):
+    return np.linalg.inv(pixels_to_metres_matrix(centre, pixel_size, shape, flip_y, rotation))
+
+
+def transform(y, x, T):
+    """T @ [y, x, 1] in f64, summed in index order (reference coordinate_transforms.py:135-140)."""
+    y = np.asarray(y, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64)
+    return (T[0, 0] * y + T[0, 1] * x) + T[0, 2], (T[1, 0] * y + T[1, 1] * x) + T[1, 2]
+
+
+def round_to_i32(v):
+    """jnp.round (half to even) then astype(int32) with XLA semantics (NaN -> 0, saturating)."""
+    r = np.rint(np.asarray(v, dtype=np.float64))
+    r = np.where(np.isnan(r), 0.0, r)
+    return np.clip(r, -2147483648.0, 2147483647.0).astype(np.int32)
+
+
+def metres_to_pixels(x, y, Tinv):
+    py, px = transform(y, x, Tinv)
+    return round_to_i32(py), round_to_i32(px)
+
+
+def into_image(py, px, shape, acc=None):
+    """Counting scatter of reference utils.py:83-114 (bounds-checked, out-of-range dropped)."""
+    h, w = int(shape[0]), int(shape[1])
+    acc = np.zeros((h, w), dtype=np.int64) if acc is None else acc
+    py = np.ravel(py).astype(np.int64)
+    px = np.ravel(px).astype(np.int64)
+    ok = (py >= 0) & (py < h) & (px >= 0) & (px < w)
+    np.add.at(acc, (py[ok], px[ok]), 1)
+    return acc
+
+
+def into_image_loop(py, px, shape):
+    """Serial per-ray loop, the literal numba algorithm (for brute-force checks)."""
+    h, w = int(shape[0]), int(shape[1])
+    acc = np.zeros((h, w), dtype=np.int64)
+    for a, b in zip(np.ravel(py), np.ravel(px)):
+        if 0 <= a < h and 0 <= b < w:
+            acc[a, b] += 1
+    return acc
+
+
+# ---- the plain run_to_end workload (lens + detector, uniform random rays) ----
+
+def plain_workload(n, seed=0, dtype=np.float32):
+    """Seeded rays: draw order x, y, dx, dy from default_rng(seed), in f64 then cast."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-0.1, 0.1, n).astype(dtype)
+    y = rng.uniform(-0.1, 0.1, n).astype(dtype)
+    dx = rng.uniform(-0.05, 0.05, n).astype(dtype)
+    dy = rng.uniform(-0.05, 0.05, n).astype(dtype)
+    return x, y, dx, dy
+
+
+PLAIN_MODEL = [("lens", 0.5, 1.0), ("detector", 1.0)]
+PLAIN_DET = dict(centre=(0.0, 0.0), pixel_size=(0.01, 0.01), shape=(128, 128))
+
+
+def plain_run(x, y, dx, dy):
+    """Plain workload through the oracle: (ray dict, int64 histogram)."""
+    r = run_to_end(make_ray(x, y, dx, dy), PLAIN_MODEL)
+    Tinv = metres_to_pixels_matrix(**PLAIN_DET)
+    py, px = metres_to_pixels(r["x"], r["y"], Tinv)
+    return r, into_image(py, px, PLAIN_DET["shape"])

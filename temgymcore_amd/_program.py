@@ -1,0 +1,544 @@
+"""Host compiler: component ``__call__`` bodies -> per-ray register programs.
+
+A vectorised ray bundle is never processed by a chain of array operations.
+Instead, the Python body of each component (``Lens.__call__``,
+``FreeSpaceParaxial.propagate``, user-defined components ...) is executed
+once on a *symbolic* ray whose tensor-valued fields are ``Sym`` objects.  Each
+arithmetic operation on a ``Sym`` appends one instruction to a trace; the
+trace is then dead-code eliminated, register-allocated and run by
+``tg_ray_program_run`` (one kernel launch for the whole component list).
+
+Scalar semantics follow the reference's JAX-with-x64 behaviour exactly
+(reference run.py:47-116 executes the same Python bodies on jnp arrays):
+
+* arithmetic between Python scalars stays in Python (double precision);
+* a Python scalar meeting a per-ray value is rounded once to the ray dtype
+  (weak typing) and the operation is carried out in that dtype;
+* ``x ** 2`` is ``x * x`` (``lax.integer_pow``);
+* every operation rounds once (no fused multiply-add).
+
+Data-dependent Python control flow on a ``Sym`` raises ``TypeError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+import numbers
+from typing import Any, Callable
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_OPC = _lib.OPCODES
+
+# Fields of a Ray that are traced; other dataclass fields (e.g. the beam
+# parameters of a GaussianRay) pass through untouched.
+RAY_FIELDS = ("x", "y", "dx", "dy", "z", "pathlength", "_one")
+
+
+def is_number(v) -> bool:
+    return isinstance(v, numbers.Number) and not isinstance(v, complex)
+
+
+def _is_array(v) -> bool:
+    return isinstance(v, (torch.Tensor, np.ndarray))
+
+
+def is_device_ray(ray) -> bool:
+    """True if any traced field of `ray` holds a (non-scalar) array."""
+    for name in RAY_FIELDS:
+        v = getattr(ray, name, None)
+        if _is_array(v) and v.ndim > 0:
+            return True
+    return False
+
+
+def to_device_tensor(v, device) -> torch.Tensor:
+    """Move an array-like onto `device`; integer/bool/half data become f64/f32."""
+    if isinstance(v, np.ndarray):
+        v = torch.from_numpy(np.ascontiguousarray(v))
+    elif not isinstance(v, torch.Tensor):
+        v = torch.as_tensor(v)
+    if not v.is_floating_point():
+        # jnp promotes an integer array meeting a weak float to the default
+        # float type, which is f64 under jax_enable_x64.
+        v = v.to(torch.float64)
+    elif v.dtype not in (torch.float32, torch.float64):
+        v = v.to(torch.float32)
+    if v.device != device:
+        v = v.to(device)
+    return v
+
+
+class Sym:
+    """A per-ray value that lives in a register of the program being traced."""
+
+    __slots__ = ("tr", "vid")
+    __array_priority__ = 1000  # make numpy scalars defer to Sym operators
+
+    def __init__(self, tr: "Trace", vid: int):
+        self.tr = tr
+        self.vid = vid
+
+    # arithmetic --------------------------------------------------------
+    def __add__(self, o):
+        return self.tr.binary("add", self, o)
+
+    def __radd__(self, o):
+        return self.tr.binary("add", o, self)
+
+    def __sub__(self, o):
+        return self.tr.binary("sub", self, o)
+
+    def __rsub__(self, o):
+        return self.tr.binary("sub", o, self)
+
+    def __mul__(self, o):
+        return self.tr.binary("mul", self, o)
+
+    def __rmul__(self, o):
+        return self.tr.binary("mul", o, self)
+
+    def __truediv__(self, o):
+        return self.tr.binary("div", self, o)
+
+    def __rtruediv__(self, o):
+        return self.tr.binary("div", o, self)
+
+    def __neg__(self):
+        return self.tr.unary("neg", self)
+
+    def __pos__(self):
+        return self
+
+    def __pow__(self, p):
+        if not is_number(p) or float(p) != int(p):
+            raise NotImplementedError("ray programs support only integer powers of per-ray values")
+        p = int(p)
+        if p == 0:
+            return 1.0
+        if p < 0:
+            return 1.0 / (self ** (-p))
+        # lax.integer_pow: repeated squaring, x**2 == x*x exactly
+        acc = None
+        base = self
+        while p:
+            if p & 1:
+                acc = base if acc is None else acc * base
+            p >>= 1
+            if p:
+                base = base * base
+        return acc
+
+    def sign(self):
+        return self.tr.unary("sign", self)
+
+    # control flow on per-ray values is not expressible ---------------------
+    def __bool__(self):
+        raise TypeError("per-ray value used in Python control flow; ray programs are straight-line")
+
+    def __float__(self):
+        raise TypeError("per-ray value converted to a Python float inside a component")
+
+    __index__ = __int__ = __float__
+
+    def __repr__(self):
+        return f"Sym(v{self.vid})"
+
+
+def sign(v):
+    """``jnp.sign`` for numbers and per-ray values."""
+    if isinstance(v, Sym):
+        return v.sign()
+    if _is_array(v):
+        return torch.sign(v) if isinstance(v, torch.Tensor) else np.sign(v)
+    if v != v or v == 0:
+        return v
+    return 1.0 if v > 0 else -1.0
+
+
+_PY = {
+    "add": lambda a, b: a + b,
+    "sub": lambda a, b: a - b,
+    "mul": lambda a, b: a * b,
+    "div": lambda a, b: a / b,
+}
+
+
+@dataclasses.dataclass
+class _Instr:
+    op: str
+    dst: int
+    a: int
+    b: int
+    imm: float
+
+
+class Trace:
+    """Records the instructions produced by symbolic execution."""
+
+    def __init__(self, device: torch.device):
+        self.device = device
+        self.instrs: list[_Instr] = []
+        self.n_values = 0
+        self.slot_tensors: list[torch.Tensor] = []
+        self.slot_value: list[int] = []
+        self._slot_by_id: dict[int, int] = {}
+        self.deps: list[frozenset] = []   # value id -> slot indices it depends on
+        self.vdtype: list[torch.dtype] = []  # value id -> dtype (per-operation promotion)
+        self.value_slot: dict[int, int] = {}
+
+    # value construction ----------------------------------------------------
+    def _new_value(self, deps: frozenset, dtype: torch.dtype) -> Sym:
+        vid = self.n_values
+        self.n_values += 1
+        self.deps.append(deps)
+        self.vdtype.append(dtype)
+        return Sym(self, vid)
+
+    def input(self, arr) -> Sym:
+        key = id(arr)
+        if key in self._slot_by_id:
+            s = self._slot_by_id[key]
+            return Sym(self, self.slot_value[s])
+        t = to_device_tensor(arr, self.device)
+        s = len(self.slot_tensors)
+        sym = self._new_value(frozenset((s,)), t.dtype)
+        self.slot_tensors.append(t)
+        self.slot_value.append(sym.vid)
+        self._slot_by_id[key] = s
+        self.value_slot[sym.vid] = s
+        return sym
+
+    def lift(self, v):
+        """Sym stays Sym; arrays become input slots; scalars stay Python numbers."""
+        if isinstance(v, Sym):
+            if v.tr is not self:
+                raise ValueError("per-ray values from two different traces were combined")
+            return v
+        if is_number(v):
+            return v
+        if _is_array(v):
+            if v.ndim == 0:
+                return v.item()
+            return self.input(v)
+        raise TypeError(f"unsupported operand of type {type(v).__name__} in a ray program")
+
+    # instruction emission --------------------------------------------------
+    def binary(self, op: str, a, b):
+        a = self.lift(a)
+        b = self.lift(b)
+        sa, sb = isinstance(a, Sym), isinstance(b, Sym)
+        if not sa and not sb:
+            return _PY[op](a, b)
+        if sa and sb:
+            dt = torch.promote_types(self.vdtype[a.vid], self.vdtype[b.vid])
+            out = self._new_value(self.deps[a.vid] | self.deps[b.vid], dt)
+            self.instrs.append(_Instr(op, out.vid, a.vid, b.vid, 0.0))
+            return out
+        if sa:  # per-ray OP scalar
+            out = self._new_value(self.deps[a.vid], self.vdtype[a.vid])
+            code = {"add": "addi", "sub": "subi", "mul": "muli", "div": "divi"}[op]
+            self.instrs.append(_Instr(code, out.vid, a.vid, a.vid, b))
+            return out
+        out = self._new_value(self.deps[b.vid], self.vdtype[b.vid])  # scalar OP per-ray
+        code = {"add": "addi", "sub": "rsubi", "mul": "muli", "div": "rdivi"}[op]
+        self.instrs.append(_Instr(code, out.vid, b.vid, b.vid, a))
+        return out
+
+    def unary(self, op: str, a: Sym) -> Sym:
+        out = self._new_value(self.deps[a.vid], self.vdtype[a.vid])
+        self.instrs.append(_Instr(op, out.vid, a.vid, a.vid, 0.0))
+        return out
+
+    # ray <-> symbols ---------------------------------------------------------
+    def symbolize(self, ray):
+        vals = {}
+        for f in dataclasses.fields(ray):
+            v = getattr(ray, f.name)
+            vals[f.name] = self.lift(v) if f.name in RAY_FIELDS else v
+        return _rebuild(ray, vals)
+
+
+def _rebuild(ray, vals: dict):
+    """Construct an object of type(ray) from a full field dict."""
+    init = {}
+    late = {}
+    for f in dataclasses.fields(ray):
+        if f.init:
+            init[f.name] = vals[f.name]
+        else:
+            late[f.name] = vals[f.name]
+    out = type(ray)(**init)
+    for k, v in late.items():
+        object.__setattr__(out, k, v)
+    return out
+
+
+# --------------------------------------------------------------------------
+# compilation
+# --------------------------------------------------------------------------
+
+@dataclasses.dataclass
+class Compiled:
+    """One kernel launch: a register program over one broadcast shape."""
+
+    shape: tuple
+    dtype: torch.dtype
+    instrs: list            # (opcode, dst_reg, a_reg, b_reg, imm)
+    slots: list             # (slot index, reg)
+    outputs: list           # (value id, reg)
+    keep_regs: dict         # value id -> reg at program end (kept-live values)
+    bin_regs: tuple | None  # (x_reg or -1, y_reg or -1)
+
+
+def _shape_of(trace: Trace, vid: int) -> tuple:
+    shapes = [tuple(trace.slot_tensors[s].shape) for s in sorted(trace.deps[vid])]
+    return tuple(torch.broadcast_shapes(*shapes)) if shapes else ()
+
+
+def compile_group(trace: Trace, out_vids: list[int], keep_vids: list[int], shape: tuple) -> Compiled:
+    """DCE + linear-scan register allocation for the values `out_vids` (stored)
+    and `keep_vids` (kept live to the end, e.g. binning positions)."""
+    live_out = set(out_vids) | set(keep_vids)
+    needed = set(live_out)
+    kept = []
+    for ins in reversed(trace.instrs):
+        if ins.dst in needed:
+            kept.append(ins)
+            needed.add(ins.a)
+            needed.add(ins.b)
+    kept.reverse()
+    slot_vids = [v for v in trace.slot_value if v in needed]
+    dtypes = {trace.vdtype[v] for v in needed}
+    dtype = torch.float64 if torch.float64 in dtypes else torch.float32
+    mixed = len(dtypes) > 1
+
+    last_use: dict[int, int] = {}
+    for i, ins in enumerate(kept):
+        last_use[ins.a] = i
+        last_use[ins.b] = i
+    end = len(kept)
+    for v in live_out:
+        last_use[v] = end
+
+    free = list(range(_lib.TG_MAX_REG - 1, -1, -1))
+    reg: dict[int, int] = {}
+
+    def alloc(v):
+        if not free:
+            raise NotImplementedError(
+                f"ray program needs more than {_lib.TG_MAX_REG} live per-ray values; split the model"
+            )
+        reg[v] = free.pop()
+
+    for v in slot_vids:
+        alloc(v)
+    slot_reg = {v: reg[v] for v in slot_vids}
+    out_instrs = []
+    for i, ins in enumerate(kept):
+        ra, rb = reg[ins.a], reg[ins.b]
+        # operands whose last use is this instruction release their register
+        # before the destination is allocated (the kernel reads a and b first)
+        for v in {ins.a, ins.b}:
+            if last_use.get(v, -1) == i and v not in live_out and v in reg:
+                free.append(reg.pop(v))
+        if ins.dst in reg:  # cannot happen in SSA form
+            raise AssertionError("value defined twice")
+        alloc(ins.dst)
+        code = _OPC[ins.op]
+        if mixed and trace.vdtype[ins.dst] == torch.float32:
+            code |= _lib.TG_OP_F32
+        out_instrs.append((code, reg[ins.dst], ra, rb, float(ins.imm)))
+        if last_use.get(ins.dst, -1) < i + 1 and ins.dst not in live_out:
+            free.append(reg.pop(ins.dst))
+    if len(out_instrs) > _lib.TG_MAX_INSTR:
+        raise _TooLong(len(out_instrs))
+    slots = [(trace.value_slot[v], slot_reg[v]) for v in slot_vids]
+    outputs = [(v, reg[v]) for v in out_vids]
+    keep_regs = {v: reg[v] for v in keep_vids}
+    return Compiled(shape=shape, dtype=dtype, instrs=out_instrs, slots=slots, outputs=outputs,
+                    keep_regs=keep_regs, bin_regs=None)
+
+
+class _TooLong(Exception):
+    pass
+
+
+def _pad_shape(shape: tuple) -> list:
+    if len(shape) > _lib.TG_MAX_DIM:
+        lead = int(np.prod(shape[: len(shape) - _lib.TG_MAX_DIM + 1]))
+        shape = (lead,) + tuple(shape[len(shape) - _lib.TG_MAX_DIM + 1:])
+    return [1] * (_lib.TG_MAX_DIM - len(shape)) + list(shape)
+
+
+def _slot_strides(t: torch.Tensor, shape: tuple) -> tuple[list, bool]:
+    """Element strides of `t` broadcast to `shape` (padded to 4 dims)."""
+    padded = _pad_shape(shape)
+    nd = len(shape)
+    if nd > _lib.TG_MAX_DIM:
+        t = t.expand(shape).contiguous().reshape(padded)
+        return list(t.stride()), True
+    tshape = [1] * (nd - t.ndim) + list(t.shape)
+    tstride = [0] * (nd - t.ndim) + list(t.stride())
+    st = [0 if tshape[k] == 1 and shape[k] != 1 else tstride[k] for k in range(nd)]
+    st = [0] * (_lib.TG_MAX_DIM - nd) + st
+    contig = t.is_contiguous() and tuple(t.shape) == tuple(shape)
+    return st, contig
+
+
+def prepare(trace: Trace, comp: Compiled, bin_spec: _lib.BinSpec | None = None):
+    """Allocate outputs and fill the C program struct for one compiled group.
+
+    Returns (Program, outputs: value id -> tensor, keep-alive tensors)."""
+    shape = tuple(comp.shape)
+    n = int(np.prod(shape)) if shape else 1
+    prog = _lib.Program()
+    prog.dtype = _lib.TG_F64 if comp.dtype == torch.float64 else _lib.TG_F32
+    prog.n = n
+    for k, s in enumerate(_pad_shape(shape)):
+        prog.shape[k] = s
+    prog.n_instr = len(comp.instrs)
+    for i, (op, d, a, b, imm) in enumerate(comp.instrs):
+        ins = prog.instr[i]
+        ins.op, ins.dst, ins.a, ins.b = op, d, a, b
+        ins.imm = imm
+        ins.immf = float(np.float32(imm))
+    keep = []
+    prog.n_slot = len(comp.slots)
+    for k, (s, r) in enumerate(comp.slots):
+        t = trace.slot_tensors[s]
+        keep.append(t)
+        st, contig = _slot_strides(t, shape)
+        sl = prog.slot[k]
+        sl.ptr = t.data_ptr()
+        for j in range(_lib.TG_MAX_DIM):
+            sl.stride[j] = st[j]
+        sl.reg = r
+        sl.contig = 1 if contig else 0
+        sl.dtype = _lib.TG_F64 if t.dtype == torch.float64 else _lib.TG_F32
+    outs = {}
+    prog.n_out = len(comp.outputs)
+    for k, (v, r) in enumerate(comp.outputs):
+        odt = trace.vdtype[v]
+        o = torch.empty(shape, dtype=odt, device=trace.device)
+        outs[v] = o
+        prog.out[k] = o.data_ptr()
+        prog.out_reg[k] = r
+        prog.out_dtype[k] = _lib.TG_F64 if odt == torch.float64 else _lib.TG_F32
+    if bin_spec is not None and comp.bin_regs is not None:
+        bin_spec.x_reg, bin_spec.y_reg = comp.bin_regs
+    return prog, outs, keep
+
+
+class Plan:
+    """A traced and compiled model application, ready to launch repeatedly.
+
+    Launching re-runs the same kernels on the same input and output buffers
+    (inputs may be refilled in place between launches).  ``launch()`` costs one
+    C call per program, so a plan is also what a HIP graph capture records.
+    """
+
+    def __init__(self, trace: Trace, programs: list, out_ray, acc=None):
+        self.trace = trace
+        self.programs = programs  # list of (Program, BinSpec | None, keepalive)
+        self.out_ray = out_ray
+        self.acc = acc
+
+    def launch(self, stream: int | None = None):
+        lib = _lib.load()
+        st = _lib.stream_ptr(self.trace.device) if stream is None else stream
+        for prog, spec, _ in self.programs:
+            status = lib.tg_ray_program_run(ctypes.byref(prog), ctypes.byref(spec) if spec is not None else None, st)
+            _lib.check(status, "tg_ray_program_run")
+        return self.out_ray
+
+
+# --------------------------------------------------------------------------
+# user-facing drivers
+# --------------------------------------------------------------------------
+
+def build_plan(trace: Trace, sym_ray, bin_request=None) -> Plan:
+    """Compile the trace into launchable programs and the output ray.
+
+    bin_request: (BinSpec, x, y) with x, y the symbolic final positions, or None.
+    """
+    fields = {f.name: getattr(sym_ray, f.name) for f in dataclasses.fields(sym_ray)}
+    stored: dict[str, int] = {}
+    for name, v in fields.items():
+        if isinstance(v, Sym):
+            if v.tr is not trace:
+                raise ValueError("output holds a per-ray value of a different trace")
+            stored[name] = v.vid
+    # values that are unchanged inputs are returned as the input arrays
+    direct = {name: vid for name, vid in stored.items() if vid in trace.value_slot}
+    computed = {name: vid for name, vid in stored.items() if name not in direct}
+    groups: dict[tuple, list] = {}
+    for name, vid in computed.items():
+        groups.setdefault(_shape_of(trace, vid), []).append(vid)
+    bin_shape = None
+    keep: list[int] = []
+    if bin_request is not None:
+        _, bx, by = bin_request
+        kv = [v.vid for v in (bx, by) if isinstance(v, Sym)]
+        bin_shape = tuple(torch.broadcast_shapes(*[_shape_of(trace, v) for v in kv])) if kv else (1,)
+        groups.setdefault(bin_shape, [])
+        keep = kv
+    results: dict[int, torch.Tensor] = {}
+    programs = []
+    for shape, vids in groups.items():
+        uniq = list(dict.fromkeys(vids))
+        is_bin = bin_request is not None and shape == bin_shape
+        comp = compile_group(trace, uniq, keep if is_bin else [], shape)
+        spec = None
+        if is_bin:
+            spec, bx, by = bin_request
+            # registers of the binning positions (or uniform values)
+            xr = comp.keep_regs[bx.vid] if isinstance(bx, Sym) else -1
+            yr = comp.keep_regs[by.vid] if isinstance(by, Sym) else -1
+            if not isinstance(bx, Sym):
+                spec.x_uniform = _uniform_as(bx, comp.dtype)
+            if not isinstance(by, Sym):
+                spec.y_uniform = _uniform_as(by, comp.dtype)
+            comp.bin_regs = (xr, yr)
+        prog, outs, keepalive = prepare(trace, comp, spec)
+        results.update(outs)
+        programs.append((prog, spec, keepalive))
+    out_vals = {}
+    for name, v in fields.items():
+        if name in direct:
+            out_vals[name] = trace.slot_tensors[trace.value_slot[direct[name]]]
+        elif name in computed:
+            out_vals[name] = results[computed[name]]
+        else:
+            out_vals[name] = v
+    return Plan(trace, programs, _rebuild(sym_ray, out_vals))
+
+
+def _materialize(trace: Trace, sym_ray, bin_request=None):
+    return build_plan(trace, sym_ray, bin_request).launch()
+
+
+def _uniform_as(v, dtype) -> float:
+    """A weakly typed scalar position as the binning kernel sees it."""
+    return float(np.float32(v)) if dtype == torch.float32 else float(v)
+
+
+def plan_fn(fn: Callable, ray, bin_request_fn: Callable | None = None) -> Plan:
+    """Trace `fn` (a Ray -> Ray function of component bodies) on a vectorised
+    ray and compile it.  If `bin_request_fn(sym_out)` is given it returns
+    (BinSpec, x, y) to bin the final positions in the same launch."""
+    device = _lib.require_device()
+    trace = Trace(device)
+    sym_in = trace.symbolize(ray)
+    sym_out = fn(sym_in)
+    req = bin_request_fn(sym_out) if bin_request_fn is not None else None
+    return build_plan(trace, sym_out, req)
+
+
+def apply_fn(fn: Callable, ray, bin_request_fn: Callable | None = None):
+    """Trace, compile and launch `fn` on a vectorised ray; returns the output ray."""
+    return plan_fn(fn, ray, bin_request_fn).launch()

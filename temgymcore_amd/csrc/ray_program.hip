@@ -1,0 +1,358 @@
+// Ray branch kernel: runs a host-compiled per-ray register program over a
+// vectorised ray bundle held as SoA arrays, optionally followed (in the same
+// pass) by detector binning.
+//
+// Replaces the eager element-wise stream of run_iter / run_to_end
+// (reference run.py:47-116 with propagator.py:51-72 and components.py) and the
+// serial numba scatter Grid.into_image -> inplace_sum (grid.py:231-280,
+// utils.py:83-114).
+//
+// Layout / mapping:
+//   * one lane owns R rays (stride 256 within a 256*R ray chunk, so each
+//     load/store instruction of a wavefront touches 64 consecutive elements);
+//   * the program is a kernel argument (kernarg segment -> scalar loads), so
+//     opcode dispatch is a uniform scalar branch and never diverges;
+//   * registers are named VGPRs selected through uniform switches (no scratch);
+//   * binning of small detectors (<= 16384 bins of int counts) goes through
+//     an LDS-privatised histogram flushed once per workgroup; larger detectors
+//     and float accumulators use device-scope atomics.
+// Arithmetic is exactly one IEEE operation per instruction, built with
+// -ffp-contract=off, so results are bit-identical to the same operation
+// sequence evaluated elementwise on the CPU.
+#include "tg_common.hpp"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int64_t kLdsBins = 16384;
+
+template <typename T>
+__device__ __forceinline__ T imm_of(const tg_instr& in);
+template <>
+__device__ __forceinline__ float imm_of<float>(const tg_instr& in) { return in.immf; }
+template <>
+__device__ __forceinline__ double imm_of<double>(const tg_instr& in) { return in.imm; }
+
+// The register file is twelve separately declared local arrays, selected by a
+// uniform switch.  One indexed array (or a struct of arrays, which is a single
+// allocation) is merged back by the optimiser into a dynamically indexed
+// scratch access; separate locals stay in VGPRs.
+#define TG_CP(dst, src) _Pragma("unroll") for (int u_ = 0; u_ < R; ++u_) dst[u_] = src[u_];
+#define TG_RD(idx, o)                                                                     \
+  switch (idx) {                                                                          \
+    case 0: TG_CP(o, r0) break;  case 1: TG_CP(o, r1) break;  case 2: TG_CP(o, r2) break;   \
+    case 3: TG_CP(o, r3) break;  case 4: TG_CP(o, r4) break;  case 5: TG_CP(o, r5) break;   \
+    case 6: TG_CP(o, r6) break;  case 7: TG_CP(o, r7) break;  case 8: TG_CP(o, r8) break;   \
+    case 9: TG_CP(o, r9) break;  case 10: TG_CP(o, r10) break; default: TG_CP(o, r11) break; \
+  }
+#define TG_WR(idx, v)                                                                     \
+  switch (idx) {                                                                          \
+    case 0: TG_CP(r0, v) break;  case 1: TG_CP(r1, v) break;  case 2: TG_CP(r2, v) break;   \
+    case 3: TG_CP(r3, v) break;  case 4: TG_CP(r4, v) break;  case 5: TG_CP(r5, v) break;   \
+    case 6: TG_CP(r6, v) break;  case 7: TG_CP(r7, v) break;  case 8: TG_CP(r8, v) break;   \
+    case 9: TG_CP(r9, v) break;  case 10: TG_CP(r10, v) break; default: TG_CP(r11, v) break; \
+  }
+
+template <typename T>
+__device__ __forceinline__ T sign_of(T a) {
+  if (a != a || a == T(0)) return a;
+  return a > T(0) ? T(1) : T(-1);
+}
+
+__device__ __forceinline__ int64_t slot_offset(int64_t lin, const int64_t (&shape)[TG_MAX_DIM],
+                                               const int64_t (&stride)[TG_MAX_DIM]) {
+  const int64_t i3 = lin % shape[3];
+  lin /= shape[3];
+  const int64_t i2 = lin % shape[2];
+  lin /= shape[2];
+  const int64_t i1 = lin % shape[1];
+  const int64_t i0 = lin / shape[1];
+  return i0 * stride[0] + i1 * stride[1] + i2 * stride[2] + i3 * stride[3];
+}
+
+// Pixel index of one position; -1 when it falls outside the detector.
+__device__ __forceinline__ int64_t bin_index(const tg_bin_spec& B, double y, double x) {
+  // Same operation order as apply_transformation's T @ [y, x, 1]
+  // (reference coordinate_transforms.py:104-140), in f64.
+  const double fy = __dadd_rn(__dadd_rn(__dmul_rn(B.m[0], y), __dmul_rn(B.m[1], x)), B.m[2]);
+  const double fx = __dadd_rn(__dadd_rn(__dmul_rn(B.m[3], y), __dmul_rn(B.m[4], x)), B.m[5]);
+  // jnp.round is round-half-to-even; rint uses the default RNE mode.
+  const int32_t iy = tg::xla_f64_to_i32(rint(fy));
+  const int32_t ix = tg::xla_f64_to_i32(rint(fx));
+  if (iy < 0 || iy >= B.height || ix < 0 || ix >= B.width) return -1;
+  return (int64_t)iy * B.width + ix;
+}
+
+template <typename T>
+__device__ __forceinline__ void global_deposit(const tg_bin_spec& B, int64_t p, T w) {
+  switch (B.acc_dtype) {
+    case 0:
+      atomicAdd((int32_t*)B.acc + p, 1);
+      break;
+    case 1:
+      atomicAdd((unsigned long long*)B.acc + p, 1ull);
+      break;
+    case 2:
+      unsafeAtomicAdd((float*)B.acc + p, (float)w);
+      break;
+    default:
+      unsafeAtomicAdd((double*)B.acc + p, (double)w);
+      break;
+  }
+}
+
+// BIN: 0 = no binning, 1 = LDS-privatised integer histogram, 2 = global atomics.
+template <typename T, int R, bool CONTIG, int BIN>
+__global__ __launch_bounds__(kThreads) void ray_program_kernel(const tg_program P, const tg_bin_spec B) {
+  extern __shared__ int32_t lds_hist[];
+  const int64_t n = P.n;
+  int64_t hw = 0;
+  if constexpr (BIN == 1) {
+    hw = B.height * B.width;
+    for (int64_t i = threadIdx.x; i < hw; i += kThreads) lds_hist[i] = 0;
+    __syncthreads();
+  }
+  const int64_t chunk = (int64_t)kThreads * R;
+  for (int64_t base = (int64_t)blockIdx.x * chunk; base < n; base += (int64_t)gridDim.x * chunk) {
+    T r0[R], r1[R], r2[R], r3[R], r4[R], r5[R], r6[R], r7[R], r8[R], r9[R], r10[R], r11[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u)
+      r0[u] = r1[u] = r2[u] = r3[u] = r4[u] = r5[u] = r6[u] = r7[u] = r8[u] = r9[u] = r10[u] = r11[u] = T(0);
+    int64_t idx[R];
+    bool ok[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      idx[u] = base + (int64_t)u * kThreads + threadIdx.x;
+      ok[u] = idx[u] < n;
+    }
+    for (int s = 0; s < P.n_slot; ++s) {
+      T v[R];
+      const bool narrow = sizeof(T) == 8 && P.slot[s].dtype == TG_F32;
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        int64_t off = idx[u];
+        if constexpr (!CONTIG) off = slot_offset(idx[u], P.shape, P.slot[s].stride);
+        if (narrow)
+          v[u] = ok[u] ? (T)((const float*)P.slot[s].ptr)[off] : T(0);
+        else
+          v[u] = ok[u] ? ((const T*)P.slot[s].ptr)[off] : T(0);
+      }
+      TG_WR(P.slot[s].reg, v)
+    }
+    for (int i = 0; i < P.n_instr; ++i) {
+      const tg_instr in = P.instr[i];
+      T a[R], b[R], d[R];
+      TG_RD(in.a, a)
+      TG_RD(in.b, b)
+      const bool f32op = sizeof(T) == 8 && (in.op & TG_OP_F32);
+      const T im = f32op ? (T)in.immf : imm_of<T>(in);
+      switch (in.op & 0x7f) {
+        case TG_OP_ADD:
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = a[u] + b[u];
+          break;
+        case TG_OP_SUB:
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = a[u] - b[u];
+          break;
+        case TG_OP_MUL:
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = a[u] * b[u];
+          break;
+        case TG_OP_DIV:
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = a[u] / b[u];
+          break;
+        case TG_OP_ADDI:
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = a[u] + im;
+          break;
+        case TG_OP_SUBI:
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = a[u] - im;
+          break;
+        case TG_OP_RSUBI:
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = im - a[u];
+          break;
+        case TG_OP_MULI:
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = a[u] * im;
+          break;
+        case TG_OP_DIVI:
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = a[u] / im;
+          break;
+        case TG_OP_RDIVI:
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = im / a[u];
+          break;
+        case TG_OP_NEG:
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = -a[u];
+          break;
+        case TG_OP_MOV:
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = a[u];
+          break;
+        case TG_OP_SET:
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = im;
+          break;
+        default:  // TG_OP_SIGN (opcodes are validated on the host)
+#pragma unroll
+          for (int u = 0; u < R; ++u) d[u] = sign_of<T>(a[u]);
+          break;
+      }
+      if (f32op) {
+#pragma unroll
+        for (int u = 0; u < R; ++u) d[u] = (T)(float)d[u];
+      }
+      TG_WR(in.dst, d)
+    }
+    for (int k = 0; k < P.n_out; ++k) {
+      T v[R];
+      TG_RD(P.out_reg[k], v)
+      if (sizeof(T) == 8 && P.out_dtype[k] == TG_F32) {
+        float* __restrict__ o = (float*)P.out[k];
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+          if (ok[u]) o[idx[u]] = (float)v[u];
+      } else {
+        T* __restrict__ o = (T*)P.out[k];
+#pragma unroll
+        for (int u = 0; u < R; ++u)
+          if (ok[u]) o[idx[u]] = v[u];
+      }
+    }
+    if constexpr (BIN != 0) {
+      T xv[R], yv[R], wv[R];
+      if (B.x_reg >= 0) {
+        TG_RD(B.x_reg, xv)
+      } else {
+#pragma unroll
+        for (int u = 0; u < R; ++u) xv[u] = (T)B.x_uniform;
+      }
+      if (B.y_reg >= 0) {
+        TG_RD(B.y_reg, yv)
+      } else {
+#pragma unroll
+        for (int u = 0; u < R; ++u) yv[u] = (T)B.y_uniform;
+      }
+      if (B.w_reg >= 0) {
+        TG_RD(B.w_reg, wv)
+      } else {
+#pragma unroll
+        for (int u = 0; u < R; ++u) wv[u] = T(1);
+      }
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        if (!ok[u]) continue;
+        const int64_t p = bin_index(B, (double)yv[u], (double)xv[u]);
+        if (p < 0) continue;
+        if constexpr (BIN == 1) {
+          atomicAdd(&lds_hist[p], 1);
+        } else {
+          global_deposit<T>(B, p, wv[u]);
+        }
+      }
+    }
+  }
+  if constexpr (BIN == 1) {
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < hw; i += kThreads) {
+      const int32_t c = lds_hist[i];
+      if (c == 0) continue;
+      if (B.acc_dtype == 0)
+        atomicAdd((int32_t*)B.acc + i, c);
+      else
+        atomicAdd((unsigned long long*)B.acc + i, (unsigned long long)(int64_t)c);
+    }
+  }
+}
+
+template <typename T, int R, bool CONTIG, int BIN>
+int launch(const tg_program& P, const tg_bin_spec& B, hipStream_t stream) {
+  const int64_t chunk = (int64_t)kThreads * R;
+  const int64_t nchunks = tg::cdiv(P.n, chunk);
+  const int cus = tg::cu_count();
+  // LDS histograms are zeroed and flushed once per workgroup, so keep the
+  // grid near two workgroups per CU; streaming variants get more waves.
+  const int64_t cap = (BIN == 1) ? 2LL * cus : 16LL * cus;
+  const int grid = (int)(nchunks < cap ? nchunks : cap);
+  const size_t lds = (BIN == 1) ? (size_t)(B.height * B.width) * sizeof(int32_t) : 0;
+  hipLaunchKernelGGL((ray_program_kernel<T, R, CONTIG, BIN>), dim3(grid), dim3(kThreads), lds, stream, P, B);
+  TG_LAUNCH_CHECK("ray_program_kernel launch");
+  return TG_OK;
+}
+
+template <typename T, int R>
+int dispatch(const tg_program& P, const tg_bin_spec& B, bool contig, int bin, hipStream_t s) {
+  if (contig) {
+    if (bin == 0) return launch<T, R, true, 0>(P, B, s);
+    if (bin == 1) return launch<T, R, true, 1>(P, B, s);
+    return launch<T, R, true, 2>(P, B, s);
+  }
+  if (bin == 0) return launch<T, R, false, 0>(P, B, s);
+  if (bin == 1) return launch<T, R, false, 1>(P, B, s);
+  return launch<T, R, false, 2>(P, B, s);
+}
+
+}  // namespace
+
+extern "C" int tg_ray_program_run(const tg_program* prog, const tg_bin_spec* bin, void* stream) {
+  tg::clear_error();
+  TG_REQUIRE(prog != nullptr, "tg_ray_program_run: prog is NULL");
+  const tg_program& P = *prog;
+  TG_REQUIRE(P.dtype == TG_F32 || P.dtype == TG_F64, "tg_ray_program_run: bad dtype %d", P.dtype);
+  TG_REQUIRE(P.n >= 0, "tg_ray_program_run: negative ray count %lld", (long long)P.n);
+  TG_REQUIRE(P.n_instr >= 0 && P.n_instr <= TG_MAX_INSTR, "tg_ray_program_run: n_instr %d out of range",
+             P.n_instr);
+  TG_REQUIRE(P.n_slot >= 0 && P.n_slot <= TG_MAX_SLOT, "tg_ray_program_run: n_slot %d out of range", P.n_slot);
+  TG_REQUIRE(P.n_out >= 0 && P.n_out <= TG_MAX_SLOT, "tg_ray_program_run: n_out %d out of range", P.n_out);
+  int64_t prod = 1;
+  for (int k = 0; k < TG_MAX_DIM; ++k) {
+    TG_REQUIRE(P.shape[k] >= 0, "tg_ray_program_run: negative shape");
+    prod *= P.shape[k];
+  }
+  TG_REQUIRE(prod == P.n, "tg_ray_program_run: prod(shape)=%lld != n=%lld", (long long)prod, (long long)P.n);
+  for (int i = 0; i < P.n_instr; ++i) {
+    const tg_instr& in = P.instr[i];
+    TG_REQUIRE((in.op & 0x7f) < TG_OP_COUNT, "tg_ray_program_run: instr %d has bad opcode %d", i, (int)in.op);
+    TG_REQUIRE(P.dtype == TG_F64 || !(in.op & TG_OP_F32), "tg_ray_program_run: f32 flag in an f32 program");
+    TG_REQUIRE(in.dst < TG_MAX_REG && in.a < TG_MAX_REG && in.b < TG_MAX_REG,
+               "tg_ray_program_run: instr %d register out of range", i);
+  }
+  bool contig = true;
+  for (int s = 0; s < P.n_slot; ++s) {
+    TG_REQUIRE(P.slot[s].reg >= 0 && P.slot[s].reg < TG_MAX_REG, "tg_ray_program_run: slot %d bad reg", s);
+    TG_REQUIRE(P.n == 0 || P.slot[s].ptr != nullptr, "tg_ray_program_run: slot %d has NULL pointer", s);
+    TG_REQUIRE(P.slot[s].dtype == P.dtype || (P.dtype == TG_F64 && P.slot[s].dtype == TG_F32),
+               "tg_ray_program_run: slot %d dtype %d incompatible with program dtype", s, P.slot[s].dtype);
+    contig = contig && (P.slot[s].contig != 0);
+  }
+  for (int k = 0; k < P.n_out; ++k) {
+    TG_REQUIRE(P.out_reg[k] >= 0 && P.out_reg[k] < TG_MAX_REG, "tg_ray_program_run: out %d bad reg", k);
+    TG_REQUIRE(P.n == 0 || P.out[k] != nullptr, "tg_ray_program_run: out %d has NULL pointer", k);
+    TG_REQUIRE(P.out_dtype[k] == P.dtype || (P.dtype == TG_F64 && P.out_dtype[k] == TG_F32),
+               "tg_ray_program_run: out %d dtype %d incompatible with program dtype", k, P.out_dtype[k]);
+  }
+  int mode = 0;
+  tg_bin_spec B{};
+  if (bin != nullptr && bin->enabled) {
+    B = *bin;
+    TG_REQUIRE(B.acc != nullptr, "tg_ray_program_run: bin accumulator is NULL");
+    TG_REQUIRE(B.height > 0 && B.width > 0, "tg_ray_program_run: bad detector shape");
+    TG_REQUIRE(B.acc_dtype >= 0 && B.acc_dtype <= 3, "tg_ray_program_run: bad accumulator dtype");
+    TG_REQUIRE(B.x_reg >= -1 && B.x_reg < TG_MAX_REG && B.y_reg >= -1 && B.y_reg < TG_MAX_REG &&
+                   B.w_reg >= -1 && B.w_reg < TG_MAX_REG,
+               "tg_ray_program_run: bin register out of range");
+    TG_REQUIRE(B.acc_dtype >= 2 || B.w_reg < 0, "tg_ray_program_run: weights need a float accumulator");
+    const bool small = B.height * B.width <= kLdsBins;
+    mode = (B.acc_dtype <= 1 && small) ? 1 : 2;
+  }
+  if (P.n == 0) return TG_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (P.dtype == TG_F32) return dispatch<float, 4>(P, B, contig, mode, s);
+  return dispatch<double, 2>(P, B, contig, mode, s);
+}

@@ -1,0 +1,209 @@
+"""GPU parity of the ray-program kernel (through the C ABI) against the oracle.
+
+Bar: bit-exact positions/slopes/pathlength for f32 and f64 bundles, and a
+bit-exact integer histogram, at the full plain workload size (1,048,576 rays)
+and on edge cases (empty, single, ragged sizes, NaN / out-of-range rays,
+broadcast parameters, mixed f32/f64, large detectors on the atomic path)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ray_oracle as O
+from temgymcore_amd.components import (
+    Biprism, Deflector, DescanError, Descanner, Detector, Lens, Plane, Rotator, Scanner,
+)
+from temgymcore_amd.ray import Ray
+from temgymcore_amd.run import plan_run_to_image, run_iter, run_to_end, run_to_image
+
+pytestmark = pytest.mark.gpu
+
+PLAIN_MODEL = [Lens(z=0.5, focal_length=1.0), Detector(z=1.0, pixel_size=(0.01, 0.01), shape=(128, 128))]
+
+
+def _dev(a, device):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+def _host(t):
+    return t.detach().cpu().numpy()
+
+
+def _plain_ray(n, device, seed=0, dtype=np.float32):
+    x, y, dx, dy = O.plain_workload(n, seed=seed, dtype=dtype)
+    ray = Ray(x=_dev(x, device), y=_dev(y, device), dx=_dev(dx, device), dy=_dev(dy, device), z=0.0, pathlength=0.0)
+    return ray, (x, y, dx, dy)
+
+
+@pytest.mark.parametrize("n", [1_048_576, 1, 1023, 1025, 4097])
+def test_plain_run_to_image_bit_exact(device, n):
+    print(f"[plain run_to_image n={n}]", flush=True)
+    ray, (x, y, dx, dy) = _plain_ray(n, device)
+    out, img = run_to_image(ray, PLAIN_MODEL)
+    torch.cuda.synchronize()
+    ref, hist = O.plain_run(x, y, dx, dy)
+    for k in ("x", "y", "dx", "dy", "pathlength"):
+        got = _host(getattr(out, k))
+        assert got.dtype == np.float32
+        np.testing.assert_array_equal(got, ref[k], err_msg=k)
+    assert out.z == 1.0
+    assert img.dtype == torch.int32
+    np.testing.assert_array_equal(_host(img).astype(np.int64), hist)
+    assert int(img.sum()) == n
+
+
+def test_run_to_end_then_into_image_matches_fused(device):
+    ray, (x, y, dx, dy) = _plain_ray(300_000, device, seed=3)
+    out = run_to_end(ray, PLAIN_MODEL)
+    img = PLAIN_MODEL[-1].into_image(out)
+    out2, img2 = run_to_image(ray, PLAIN_MODEL)
+    torch.cuda.synchronize()
+    assert torch.equal(img, img2)
+    for k in ("x", "y", "pathlength"):
+        assert torch.equal(getattr(out, k), getattr(out2, k))
+
+
+def test_plan_relaunch_is_deterministic(device):
+    ray, (x, y, dx, dy) = _plain_ray(65_536, device, seed=4)
+    plan = plan_run_to_image(ray, PLAIN_MODEL)
+    first = None
+    for _ in range(3):
+        plan.acc.zero_()
+        plan.launch()
+        cur = plan.acc.clone()
+        if first is None:
+            first = cur
+        assert torch.equal(cur, first)
+    _, hist = O.plain_run(x, y, dx, dy)
+    np.testing.assert_array_equal(_host(first).astype(np.int64), hist)
+
+
+def test_run_iter_per_step_bit_exact(device):
+    ray, (x, y, dx, dy) = _plain_ray(10_000, device, seed=5)
+    steps = list(run_iter(ray, PLAIN_MODEL))
+    refs = O.run_steps(O.make_ray(x, y, dx, dy), [("lens", 0.5, 1.0), ("detector", 1.0)])
+    assert len(steps) == len(refs) == 4
+    for (_, got), ref in zip(steps, refs):
+        for k in ("x", "y", "dx", "dy", "pathlength"):
+            g = getattr(got, k)
+            g = _host(g) if isinstance(g, torch.Tensor) else g
+            np.testing.assert_array_equal(np.asarray(g), np.asarray(ref[k]), err_msg=k)
+
+
+def _rich(err):
+    model = [
+        Scanner(z=0.1, scan_pos_x=0.003, scan_pos_y=-0.002, scan_tilt_x=1e-3, scan_tilt_y=-2e-3),
+        Deflector(z=0.2, def_x=0.01, def_y=-0.02),
+        Lens(z=0.35, focal_length=0.7),
+        Rotator(z=0.4, angle=12.0),
+        Descanner(z=0.5, scan_pos_x=0.003, scan_pos_y=-0.002, scan_tilt_x=1e-3, scan_tilt_y=-2e-3,
+                  descan_error=DescanError(*err)),
+        Biprism(z=0.55, def_x=1e-3),
+        Plane(z=0.6),
+        Detector(z=1.0, pixel_size=(0.002, 0.002), shape=(256, 256)),
+    ]
+    omodel = [
+        ("scanner", 0.1, 0.003, -0.002, 1e-3, -2e-3), ("deflector", 0.2, 0.01, -0.02),
+        ("lens", 0.35, 0.7), ("rotator", 0.4, 12.0),
+        ("descanner", 0.5, 0.003, -0.002, 1e-3, -2e-3, err), ("biprism", 0.55, 1e-3),
+        ("plane", 0.6), ("detector", 1.0),
+    ]
+    return model, omodel
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_rich_model_bit_exact_large_detector(device, dtype):
+    print(f"[rich model dtype={dtype.__name__}]", flush=True)
+    rng = np.random.default_rng(7)
+    n = 200_003
+    x, y = (rng.uniform(-0.1, 0.1, n).astype(dtype) for _ in range(2))
+    dx, dy = (rng.uniform(-0.05, 0.05, n).astype(dtype) for _ in range(2))
+    pl = rng.uniform(0, 1e-3, n).astype(dtype)
+    err = tuple(float(v) for v in rng.normal(0, 1e-3, 12))
+    model, omodel = _rich(err)
+    ray = Ray(x=_dev(x, device), y=_dev(y, device), dx=_dev(dx, device), dy=_dev(dy, device), z=0.05,
+              pathlength=_dev(pl, device))
+    acc = torch.zeros((256, 256), dtype=torch.int64, device=device)
+    out, img = run_to_image(ray, model, acc=acc)
+    torch.cuda.synchronize()
+    ref = O.run_to_end(O.make_ray(x, y, dx, dy, 0.05, pl), omodel)
+    for k in ("x", "y", "dx", "dy", "pathlength"):
+        np.testing.assert_array_equal(_host(getattr(out, k)), ref[k], err_msg=k)
+    Tinv = O.metres_to_pixels_matrix((0.0, 0.0), (0.002, 0.002), (256, 256))
+    py, px = O.metres_to_pixels(ref["x"], ref["y"], Tinv)
+    np.testing.assert_array_equal(_host(img), O.into_image(py, px, (256, 256)))
+    assert img.data_ptr() == acc.data_ptr()
+
+
+def test_nan_and_out_of_range_rays(device):
+    # NaN positions cast to pixel 0 (XLA float->int), huge ones saturate and drop
+    x = np.array([np.nan, 0.0, 1e30, -1e30, 0.635, -0.64, 0.005, 0.015], dtype=np.float32)
+    y = np.array([0.0, np.nan, 0.0, 0.0, 0.0, 0.0, 0.6349, -0.6351], dtype=np.float32)
+    det = Detector(z=0.0, pixel_size=(0.01, 0.01), shape=(128, 128))
+    img = det.into_image(Ray(x=_dev(x, device), y=_dev(y, device), dx=0.0, dy=0.0, z=0.0, pathlength=0.0))
+    torch.cuda.synchronize()
+    Tinv = O.metres_to_pixels_matrix((0.0, 0.0), (0.01, 0.01), (128, 128))
+    py, px = O.metres_to_pixels(x, y, Tinv)
+    np.testing.assert_array_equal(_host(img).astype(np.int64), O.into_image(py, px, (128, 128)))
+
+
+def test_empty_bundle(device):
+    z = torch.zeros(0, dtype=torch.float32, device=device)
+    out, img = run_to_image(Ray(x=z, y=z, dx=z, dy=z, z=0.0, pathlength=0.0), PLAIN_MODEL)
+    torch.cuda.synchronize()
+    assert out.x.numel() == 0 and int(img.sum()) == 0
+
+
+def test_broadcast_mixed_precision(device):
+    S, J = 33, 65
+    rng = np.random.default_rng(2)
+    spx = rng.uniform(-1e-9, 1e-9, (S, 1))
+    spy = rng.uniform(-1e-9, 1e-9, (S, 1))
+    dx = rng.uniform(-5e-4, 5e-4, (1, J)).astype(np.float32)
+    dy = rng.uniform(-5e-4, 5e-4, (1, J)).astype(np.float32)
+    model = [Scanner(z=0.0, scan_pos_x=_dev(spx, device), scan_pos_y=_dev(spy, device)),
+             Plane(z=2e-7),
+             Descanner(z=2e-7, scan_pos_x=_dev(spx, device), scan_pos_y=_dev(spy, device)),
+             Detector(z=0.5, pixel_size=(5e-6, 5e-6), shape=(256, 256))]
+    out = run_to_end(Ray(x=0.0, y=0.0, dx=_dev(dx, device), dy=_dev(dy, device), z=0.0, pathlength=0.0), model)
+    torch.cuda.synchronize()
+    om = [("scanner", 0.0, spx, spy, 0.0, 0.0), ("plane", 2e-7),
+          ("descanner", 2e-7, spx, spy, 0.0, 0.0, (0.0,) * 12), ("detector", 0.5)]
+    ref = O.run_to_end(O.make_ray(0.0, 0.0, dx, dy), om)
+    assert tuple(out.x.shape) == (S, J) and out.x.dtype == torch.float64
+    np.testing.assert_array_equal(_host(out.x), ref["x"])
+    np.testing.assert_array_equal(_host(out.y), ref["y"])
+    assert tuple(out.dx.shape) == (1, J)
+
+
+def test_user_defined_component(device):
+    import dataclasses
+
+    from temgymcore_amd.components import Component
+
+    @dataclasses.dataclass(frozen=True)
+    class Wedge(Component):
+        z: float
+        k: float
+
+        def __call__(self, ray):
+            return ray.derive(dx=ray.dx + self.k * ray.x * ray.x, pathlength=ray.pathlength + ray.x / self.k)
+
+    rng = np.random.default_rng(9)
+    x = rng.uniform(-1, 1, 5000).astype(np.float32)
+    dx = rng.uniform(-1, 1, 5000).astype(np.float32)
+    out = run_to_end(Ray(x=_dev(x, device), y=0.0, dx=_dev(dx, device), dy=0.0, z=0.0, pathlength=0.0),
+                     [Wedge(z=0.25, k=3.0)])
+    torch.cuda.synchronize()
+    xp = x + dx * np.float32(0.25)
+    np.testing.assert_array_equal(_host(out.dx), dx + np.float32(3.0) * xp * xp)
+    np.testing.assert_array_equal(_host(out.pathlength), np.float32(0.25) + xp / np.float32(3.0))
+
+
+def test_direct_component_call_on_device_ray(device):
+    ray, (x, y, dx, dy) = _plain_ray(777, device, seed=8)
+    out = Lens(z=0.0, focal_length=0.3)(ray)
+    torch.cuda.synchronize()
+    ref = O.lens(O.make_ray(x, y, dx, dy), 0.3)
+    for k in ("dx", "dy", "pathlength"):
+        np.testing.assert_array_equal(_host(getattr(out, k)), ref[k])
+    assert out.x is ray.x  # unchanged fields are passed through

@@ -1,0 +1,276 @@
+"""Host-side checks (no GPU): the component tracer / register allocator, the
+library's exported symbols, and the scalar-ray API against the oracle.
+
+The compiled register programs are executed here by a small numpy
+interpreter that applies one IEEE operation per instruction, exactly as the
+kernel does; its results must be bit-identical to the oracle's f32 run."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ray_oracle as O
+from temgymcore_amd import _lib, _program
+from temgymcore_amd.components import (
+    Biprism, Deflector, DescanError, Descanner, Detector, Lens, Plane, Rotator, Scanner,
+)
+from temgymcore_amd.ray import Ray
+from temgymcore_amd.run import _trace_model, run_iter, run_to_end, solve_model
+from temgymcore_amd.propagator import FreeSpaceParaxial
+from temgymcore_amd.transfer import custom_jacobian_matrix, jacobian, ray_matrix
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INV_OPC = {v: k for k, v in _lib.OPCODES.items()}
+
+
+def interpret(trace, comp):
+    """numpy execution of a Compiled program (test infrastructure)."""
+    dt = np.float32 if comp.dtype == torch.float32 else np.float64
+    shape = tuple(comp.shape)
+    regs = [np.zeros(shape, dt) for _ in range(_lib.TG_MAX_REG)]
+    for s, r in comp.slots:
+        regs[r] = np.broadcast_to(trace.slot_tensors[s].numpy().astype(dt), shape).copy()
+    for op, d, a, b, imm in comp.instrs:
+        narrow = bool(op & _lib.TG_OP_F32)
+        op &= 0x7F
+        im = np.float64(np.float32(imm)) if narrow else dt(imm)
+        A, B = regs[a], regs[b]
+        name = INV_OPC[op]
+        with np.errstate(all="ignore"):
+            val = {
+                "add": lambda: A + B, "sub": lambda: A - B, "mul": lambda: A * B, "div": lambda: A / B,
+                "addi": lambda: A + im, "subi": lambda: A - im, "rsubi": lambda: im - A,
+                "muli": lambda: A * im, "divi": lambda: A / im, "rdivi": lambda: im / A,
+                "neg": lambda: -A, "mov": lambda: A.copy(), "set": lambda: np.full(shape, im, dt),
+                "sign": lambda: np.sign(A),
+            }[name]()
+        regs[d] = val.astype(np.float32).astype(dt) if narrow else val.astype(dt)
+    return {v: regs[r].astype(np.float32 if trace.vdtype[v] == torch.float32 else dt) for v, r in comp.outputs}
+
+
+def run_host(ray, model):
+    """Trace + compile `model` on CPU tensors and interpret: dict of outputs."""
+    tr = _program.Trace(torch.device("cpu"))
+    out = _trace_model(tr.symbolize(ray), model, FreeSpaceParaxial())
+    res = {}
+    groups = {}
+    for name in ("x", "y", "dx", "dy", "z", "pathlength", "_one"):
+        v = getattr(out, name)
+        if isinstance(v, _program.Sym):
+            if v.vid in tr.value_slot:
+                res[name] = tr.slot_tensors[tr.value_slot[v.vid]].numpy()
+            else:
+                groups.setdefault(_program._shape_of(tr, v.vid), []).append((name, v.vid))
+        else:
+            res[name] = v
+    for shape, items in groups.items():
+        comp = _program.compile_group(tr, [vid for _, vid in items], [], shape)
+        vals = interpret(tr, comp)
+        for name, vid in items:
+            res[name] = vals[vid]
+    return res, tr
+
+
+def test_plain_model_bit_exact_vs_oracle():
+    x, y, dx, dy = O.plain_workload(50000, seed=0)
+    ref, _ = O.plain_run(x, y, dx, dy)
+    ray = Ray(x=torch.from_numpy(x), y=torch.from_numpy(y), dx=torch.from_numpy(dx), dy=torch.from_numpy(dy),
+              z=0.0, pathlength=0.0)
+    model = [Lens(z=0.5, focal_length=1.0), Detector(z=1.0, pixel_size=(0.01, 0.01), shape=(128, 128))]
+    res, _ = run_host(ray, model)
+    for k in ("x", "y", "dx", "dy", "pathlength"):
+        assert res[k].dtype == np.float32
+        np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
+    assert res["z"] == ref["z"] == 1.0
+
+
+def _rich_model(err):
+    return [
+        Scanner(z=0.1, scan_pos_x=0.003, scan_pos_y=-0.002, scan_tilt_x=1e-3, scan_tilt_y=-2e-3),
+        Deflector(z=0.2, def_x=0.01, def_y=-0.02),
+        Lens(z=0.35, focal_length=0.7),
+        Rotator(z=0.4, angle=12.0),
+        Descanner(z=0.5, scan_pos_x=0.003, scan_pos_y=-0.002, scan_tilt_x=1e-3, scan_tilt_y=-2e-3,
+                  descan_error=DescanError(*err)),
+        Biprism(z=0.55, def_x=1e-3),
+        Plane(z=0.6),
+        Detector(z=1.0, pixel_size=(0.01, 0.01), shape=(64, 64)),
+    ]
+
+
+def _rich_oracle(err):
+    return [
+        ("scanner", 0.1, 0.003, -0.002, 1e-3, -2e-3), ("deflector", 0.2, 0.01, -0.02),
+        ("lens", 0.35, 0.7), ("rotator", 0.4, 12.0),
+        ("descanner", 0.5, 0.003, -0.002, 1e-3, -2e-3, err), ("biprism", 0.55, 1e-3),
+        ("plane", 0.6), ("detector", 1.0),
+    ]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_rich_model_bit_exact_vs_oracle(dtype):
+    rng = np.random.default_rng(7)
+    n = 4097
+    x, y = (rng.uniform(-0.05, 0.05, n).astype(dtype) for _ in range(2))
+    dx, dy = (rng.uniform(-0.02, 0.02, n).astype(dtype) for _ in range(2))
+    pl = rng.uniform(0, 1e-3, n).astype(dtype)
+    err = tuple(float(v) for v in rng.normal(0, 1e-3, 12))
+    ray = Ray(x=torch.from_numpy(x), y=torch.from_numpy(y), dx=torch.from_numpy(dx), dy=torch.from_numpy(dy),
+              z=0.05, pathlength=torch.from_numpy(pl))
+    res, _ = run_host(ray, _rich_model(err))
+    ref = O.run_to_end(O.make_ray(x, y, dx, dy, 0.05, pl), _rich_oracle(err))
+    for k in ("x", "y", "dx", "dy", "pathlength"):
+        assert res[k].dtype == dtype
+        np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
+
+
+def test_point_source_scalar_positions_weak_typing():
+    # x, y uniform (Python floats), slopes per ray: the first propagation must
+    # round the scalar x once to f32 and add fl(dx*d) -- as the oracle does.
+    rng = np.random.default_rng(1)
+    dx = rng.uniform(-1e-3, 1e-3, 999).astype(np.float32)
+    dy = rng.uniform(-1e-3, 1e-3, 999).astype(np.float32)
+    ray = Ray(x=1e-9, y=-3e-9, dx=torch.from_numpy(dx), dy=torch.from_numpy(dy), z=0.0, pathlength=0.0)
+    model = [Lens(z=2e-7, focal_length=1e-3), Detector(z=0.5, pixel_size=(1e-5, 1e-5), shape=(32, 32))]
+    res, _ = run_host(ray, model)
+    ref = O.run_to_end(O.make_ray(1e-9, -3e-9, dx, dy), [("lens", 2e-7, 1e-3), ("detector", 0.5)])
+    for k in ("x", "y", "dx", "dy", "pathlength"):
+        np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
+
+
+def test_broadcast_scan_positions():
+    # per-scan-position parameters [S,1] against per-ray slopes [1,J]
+    S, J = 5, 7
+    rng = np.random.default_rng(2)
+    spx = rng.uniform(-1e-9, 1e-9, (S, 1))
+    dx = rng.uniform(-1e-3, 1e-3, (1, J)).astype(np.float32)
+    ray = Ray(x=0.0, y=0.0, dx=torch.from_numpy(dx), dy=torch.from_numpy(dx.copy()), z=0.0, pathlength=0.0)
+    model = [Scanner(z=0.0, scan_pos_x=torch.from_numpy(spx), scan_pos_y=0.0), Plane(z=1e-7)]
+    res, _ = run_host(ray, model)
+    assert res["x"].shape == (S, J) and res["x"].dtype == np.float64  # f64 params promote
+    ref = O.run_to_end(O.make_ray(0.0, 0.0, dx, dx.copy()), [("scanner", 0.0, spx, 0.0, 0.0, 0.0), ("plane", 1e-7)])
+    np.testing.assert_array_equal(res["x"], ref["x"])
+    assert res["dx"].shape == (1, J)  # unchanged field keeps its own shape
+
+
+def test_register_pressure_and_dce():
+    x = torch.zeros(10, dtype=torch.float32)
+    ray = Ray(x=x, y=x.clone(), dx=x.clone(), dy=x.clone(), z=0.0, pathlength=0.0)
+    model = [Lens(z=0.1 * (i + 1), focal_length=1.0 + i) for i in range(6)]
+    tr = _program.Trace(torch.device("cpu"))
+    out = _trace_model(tr.symbolize(ray), model, FreeSpaceParaxial())
+    comp = _program.compile_group(tr, [out.x.vid], [], (10,))
+    full = _program.compile_group(tr, [out.x.vid, out.pathlength.vid], [], (10,))
+    assert len(comp.instrs) < len(full.instrs)  # pathlength work eliminated
+    regs = {r for ins in full.instrs for r in ins[1:4]}
+    assert max(regs) < _lib.TG_MAX_REG
+
+
+def test_control_flow_on_per_ray_value_raises():
+    class Bad(Plane):
+        def __call__(self, ray):
+            if ray.x > 0:
+                return ray
+            return ray
+
+    tr = _program.Trace(torch.device("cpu"))
+    ray = tr.symbolize(Ray(x=torch.zeros(3), y=0.0, dx=0.0, dy=0.0, z=0.0, pathlength=0.0))
+    with pytest.raises(TypeError):
+        Bad(z=0.0)(ray)
+
+
+def test_scalar_api_matches_oracle_and_readme():
+    r = Ray(x=0.1, y=0.2, dx=0.3, dy=0.4, z=0.0, pathlength=0.0)
+    model = [Lens(z=0.5, focal_length=1.0), Detector(z=1.0, pixel_size=(0.01, 0.01), shape=(128, 128))]
+    out = run_to_end(r, model)
+    ref = O.run_to_end(O.make_ray(0.1, 0.2, 0.3, 0.4), [("lens", 0.5, 1.0), ("detector", 1.0)])
+    assert (out.x, out.y, out.dx, out.dy, out.z, out.pathlength) == (
+        ref["x"], ref["y"], ref["dx"], ref["dy"], ref["z"], ref["pathlength"])
+    np.testing.assert_allclose(ray_matrix(model, r), [
+        [0.5, 0, 0.75, 0, 0], [0, 0.5, 0, 0.75, 0], [-1, 0, 0.5, 0, 0], [0, -1, 0, 0.5, 0], [0, 0, 0, 0, 1]])
+    # per-step matrices (reference README.md:245-267)
+    steps = solve_model(r, model)
+    assert steps.shape == (4, 5, 5)
+    np.testing.assert_allclose(steps[0][0, 2], 0.5)
+    np.testing.assert_allclose(steps[1][2, 0], -1.0)
+
+
+def test_run_iter_step_structure():
+    # reference test_run.py:10-47: a propagator step before every element, even at d = 0
+    comps = (Scanner(z=1.2, scan_pos_x=0.0, scan_pos_y=0.0), Plane(z=1.2),
+             Descanner(z=1.2, scan_pos_x=0.0, scan_pos_y=0.0), Plane(z=3.1))
+    ray = Ray(x=0.12, y=0.23, dx=0.34, dy=0.45, z=3.14, pathlength=0.34)
+    res = list(run_iter(ray, comps))
+    assert len(res) == 2 * len(comps)
+    prev = ray
+    for i, c in enumerate(comps):
+        prop, pr = res[2 * i]
+        assert prop.distance == c.z - prev.z
+        assert pr.x == pytest.approx(prev.x + prev.dx * prop.distance)
+        prev = res[2 * i + 1][1]
+
+
+def test_descanner_jacobian_dual():
+    e = np.random.default_rng(3).random(12)
+    d = Descanner(z=0.0, scan_pos_x=1.5, scan_pos_y=-2.0, descan_error=DescanError(*e))
+    J = custom_jacobian_matrix(jacobian(d)(Ray(0.0, 0.0, 0.0, 0.0, 0.0, 0.0)))
+    T = np.eye(5)
+    T[0, 4] = 1.5 * e[0] - 2.0 * e[1] + e[8] - 1.5
+    T[1, 4] = 1.5 * e[2] - 2.0 * e[3] + e[9] + 2.0
+    T[2, 4] = 1.5 * e[4] - 2.0 * e[5] + e[10]
+    T[3, 4] = 1.5 * e[6] - 2.0 * e[7] + e[11]
+    np.testing.assert_allclose(J, T, atol=1e-12)
+
+
+def test_scan_grid_tables_host():
+    from temgymcore_amd.components import ScanGrid
+    for xy, rot, expected in [((-0.5, 0.5), 90.0, (10, 0)), ((0.5, -0.5), 0.0, (10, 10))]:
+        g = ScanGrid(z=0.0, rotation=rot, pixel_size=(0.1, 0.1), shape=(11, 11))
+        py, px = g.metres_to_pixels(xy)
+        assert (int(py), int(px)) == expected
+
+
+# ---- C ABI library ---------------------------------------------------------
+
+def _declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^[A-Za-z_][\w \*]*?\b(tg_\w+)\s*\(", txt, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared_symbols()
+    assert "tg_ray_program_run" in names
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert set(_lib._SIGNATURES) >= names
+
+
+def test_struct_layout_matches_header(tmp_path):
+    """ctypes mirrors of the ABI structs have the C compiler's layout."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "sz.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "temgym_hip.h"\n'
+        'int main(){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(tg_instr), sizeof(tg_slot),'
+        ' sizeof(tg_program), sizeof(tg_bin_spec), offsetof(tg_program, out_dtype), offsetof(tg_bin_spec, acc));}'
+    )
+    exe = tmp_path / "sz"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == [ctypes.sizeof(_lib.Instr), ctypes.sizeof(_lib.Slot), ctypes.sizeof(_lib.Program),
+                   ctypes.sizeof(_lib.BinSpec), _lib.Program.out_dtype.offset, _lib.BinSpec.acc.offset]

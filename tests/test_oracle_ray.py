@@ -1,0 +1,198 @@
+"""Pins the ray-branch oracle against the reference's own golden values,
+known-answer tables and closed forms, plus brute force for binning."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import ray_oracle as O
+
+
+def test_readme_golden_single_ray():
+    # reference README.md:35-52: Ray(0.1, 0.2, 0.3, 0.4, 0, 0) -> Lens(0.5, f=1) -> Detector(1.0)
+    r = O.run_to_end(O.make_ray(0.1, 0.2, 0.3, 0.4), [("lens", 0.5, 1.0), ("detector", 1.0)])
+    assert r["x"] == pytest.approx(0.275, abs=1e-15)
+    assert r["y"] == pytest.approx(0.4, abs=1e-15)
+    assert r["dx"] == pytest.approx(0.05, abs=1e-15)
+    assert r["dy"] == pytest.approx(0.0, abs=1e-15)
+    assert r["z"] == 1.0
+    # printed 0.89 in the README; by hand 0.5 - 0.25/2 + 0.5 + 0.5*... = 0.88875
+    assert r["pathlength"] == pytest.approx(0.88875, abs=1e-15)
+
+
+def _fd_matrix(model, r0):
+    """5x5 ABCD by central differences (x, y, dx, dy, one)."""
+    keys = ("x", "y", "dx", "dy", "one")
+    J = np.zeros((5, 5))
+    for j, k in enumerate(keys):
+        h = 1e-6
+        a = dict(r0)
+        b = dict(r0)
+        a[k] = r0[k] + h
+        b[k] = r0[k] - h
+        ra, rb = O.run_to_end(a, model), O.run_to_end(b, model)
+        for i, kk in enumerate(keys):
+            J[i, j] = (ra[kk] - rb[kk]) / (2 * h)
+    return J
+
+
+def test_readme_abcd_matrix():
+    # reference README.md:230-234
+    expected = np.array([
+        [0.5, 0, 0.75, 0, 0], [0, 0.5, 0, 0.75, 0], [-1, 0, 0.5, 0, 0], [0, -1, 0, 0.5, 0], [0, 0, 0, 0, 1]])
+    J = _fd_matrix([("lens", 0.5, 1.0), ("detector", 1.0)], O.make_ray(0.1, 0.2, 0.3, 0.4))
+    np.testing.assert_allclose(J, expected, atol=1e-8)
+
+
+def test_readme_parameter_gradient():
+    # reference README.md:124-136: d x_out / d (f, z) = (0.125, 0.1)
+    def x_out(f, z):
+        return O.run_to_end(O.make_ray(0.1, 0.2, 0.3, 0.4), [("lens", z, f), ("detector", 1.0)])["x"]
+
+    h = 1e-6
+    assert (x_out(1 + h, 0.5) - x_out(1 - h, 0.5)) / (2 * h) == pytest.approx(0.125, abs=1e-8)
+    assert (x_out(1, 0.5 + h) - x_out(1, 0.5 - h)) / (2 * h) == pytest.approx(0.1, abs=1e-8)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_propagate_closed_form(seed):
+    # reference test_rays.py:26-39
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(0.1, 10)
+    x0, y0, dx0, dy0, z0, pl0 = rng.uniform(-10, 10, 6)
+    r = O.propagate(O.make_ray(x0, y0, dx0, dy0, z0, pl0), d)
+    assert r["x"] == pytest.approx(x0 + dx0 * d, abs=1e-12)
+    assert r["y"] == pytest.approx(y0 + dy0 * d, abs=1e-12)
+    assert r["z"] == pytest.approx(z0 + d, abs=1e-12)
+    assert r["pathlength"] == pytest.approx(pl0 + d, abs=1e-12)
+
+
+def test_propagation_matrix():
+    # reference test_rays.py:82-103: Jacobian of propagation = free-space 5x5
+    d = 2.5
+    T = np.eye(5)
+    T[0, 2] = T[1, 3] = d
+    r0 = O.make_ray(0.5, -0.5, 0.1, -0.2, 1.0)
+    J = _fd_matrix([("plane", 1.0 + d)], r0)
+    np.testing.assert_allclose(J, T, atol=1e-8)
+
+
+def test_scanner_and_descanner_closed_form():
+    # reference test_component.py:186-259
+    rng = np.random.default_rng(11)
+    spx, spy = rng.uniform(-5, 5, 2)
+    stx, sty = rng.uniform(-0.5, 0.5, 2)
+    x, y, dx, dy = rng.uniform(-5, 5, 4)
+    e = rng.random(12)
+    r = O.scanner(O.make_ray(x, y, dx, dy), spx, spy, stx, sty)
+    assert (r["x"], r["y"], r["dx"], r["dy"]) == pytest.approx((x + spx, y + spy, dx + stx, dy + sty))
+    r = O.descanner(O.make_ray(x, y, dx, dy), spx, spy, stx, sty, e)
+    assert r["x"] == pytest.approx(x + spx * e[0] + spy * e[1] + e[8] - spx)
+    assert r["y"] == pytest.approx(y + spx * e[2] + spy * e[3] + e[9] - spy)
+    assert r["dx"] == pytest.approx(dx + spx * e[4] + spy * e[5] + e[10] - stx)
+    assert r["dy"] == pytest.approx(dy + spx * e[6] + spy * e[7] + e[11] - sty)
+
+
+def test_descanner_matrix_column():
+    # reference test_component.py:313-353: 5th column K1..K4, identity elsewhere
+    e = np.random.default_rng(3).random(12)
+    spx, spy = 1.5, -2.0
+    J = _fd_matrix([("descanner", 0.0, spx, spy, 0.0, 0.0, e)], O.make_ray(0.0, 0.0, 0.0, 0.0))
+    T = np.eye(5)
+    T[0, 4] = spx * e[0] + spy * e[1] + e[8] - spx
+    T[1, 4] = spx * e[2] + spy * e[3] + e[9] - spy
+    T[2, 4] = spx * e[4] + spy * e[5] + e[10]
+    T[3, 4] = spx * e[6] + spy * e[7] + e[11]
+    np.testing.assert_allclose(J, T, atol=1e-7)
+
+
+def test_lens_matrix_and_pathlength():
+    # thin lens ABCD (tests/transfer_matrices.py lens matrix) and the paraxial phase
+    f = 0.7
+    J = _fd_matrix([("lens", 0.0, f)], O.make_ray(0.01, 0.02, 0.0, 0.0))
+    T = np.eye(5)
+    T[2, 0] = T[3, 1] = -1 / f
+    np.testing.assert_allclose(J, T, atol=1e-8)
+    r = O.lens(O.make_ray(0.3, 0.4, 0.0, 0.0), f)
+    assert r["pathlength"] == pytest.approx(-(0.25) / (2 * f))
+
+
+def test_rotator_and_deflector():
+    r = O.rotator(O.make_ray(1.0, 0.0, 0.0, 1.0), 90.0)
+    assert (r["x"], r["y"], r["dx"], r["dy"]) == pytest.approx((0.0, 1.0, -1.0, 0.0), abs=1e-15)
+    r = O.deflector(O.make_ray(2.0, 3.0, 0.1, 0.2, pathlength=1.0), 0.5, -0.5)
+    assert (r["dx"], r["dy"]) == pytest.approx((0.6, -0.3))
+    assert r["pathlength"] == pytest.approx(1.0 + 0.1 * 2.0 + 0.2 * 3.0)  # incoming slopes
+
+
+TABLE_M2P = [
+    ((0.0, 0.0), 0.0, (5, 5)), ((-0.5, 0.5), 0.0, (0, 0)), ((0.5, -0.5), 0.0, (10, 10)),
+    ((0.0, 0.5), 0.0, (0, 5)), ((-0.5, 0.0), 0.0, (5, 0)),
+    ((0.0, 0.0), 90.0, (5, 5)), ((-0.5, 0.5), 90.0, (10, 0)), ((0.5, -0.5), 90.0, (0, 10)),
+    ((0.0, 0.5), 90.0, (5, 0)), ((-0.5, 0.0), 90.0, (10, 5)),
+]
+
+
+@pytest.mark.parametrize("xy, rot, expected", TABLE_M2P)
+def test_metres_to_pixels_table(xy, rot, expected):
+    # reference test_component.py:73-99 (ScanGrid) and :131-150 (Detector, rot 0)
+    Tinv = O.metres_to_pixels_matrix((0.0, 0.0), (0.1, 0.1), (11, 11), rotation=rot)
+    py, px = O.metres_to_pixels(xy[0], xy[1], Tinv)
+    assert (int(py), int(px)) == expected
+
+
+@pytest.mark.parametrize("xy, rot, px", TABLE_M2P)
+def test_pixels_to_metres_table(xy, rot, px):
+    # reference test_component.py:102-128, :153-174
+    T = O.pixels_to_metres_matrix((0.0, 0.0), (0.1, 0.1), (11, 11), rotation=rot)
+    my, mx = O.transform(px[0], px[1], T)
+    np.testing.assert_allclose((mx, my), xy, atol=1e-12)
+
+
+@pytest.mark.parametrize("rot", [-137.0, -12.0, 12.0, 75.0, 170.0])
+def test_scan_step_vector(rot):
+    # reference test_component.py:356-378: one x-pixel step = (cos t * s, -sin t * s)
+    T = O.pixels_to_metres_matrix((0.0, 0.0), (0.1, 0.1), (11, 11), rotation=rot)
+    y0, x0 = O.transform(5, 5, T)
+    y1, x1 = O.transform(5, 6, T)
+    t = math.radians(rot)
+    np.testing.assert_allclose((x1 - x0, y1 - y0), (0.1 * math.cos(t), -0.1 * math.sin(t)), atol=1e-12)
+
+
+def test_round_half_even_and_cast():
+    assert list(O.round_to_i32([0.5, 1.5, 2.5, -0.5, -1.5])) == [0, 2, 2, 0, -2]
+    assert list(O.round_to_i32([np.nan, 1e30, -1e30])) == [0, 2147483647, -2147483648]
+
+
+def test_binning_brute_force():
+    rng = np.random.default_rng(5)
+    py = rng.integers(-3, 12, 4000)
+    px = rng.integers(-3, 12, 4000)
+    a = O.into_image(py, px, (9, 10))
+    b = O.into_image_loop(py, px, (9, 10))
+    np.testing.assert_array_equal(a, b)
+    inside = ((py >= 0) & (py < 9) & (px >= 0) & (px < 10)).sum()
+    assert a.sum() == inside
+
+
+def test_plain_workload_histogram_consistency():
+    x, y, dx, dy = O.plain_workload(20000, seed=0)
+    r, hist = O.plain_run(x, y, dx, dy)
+    assert r["x"].dtype == np.float32 and r["pathlength"].dtype == np.float32
+    # x_det = 0.5 x + 0.75 dx (README ABCD): f32 result within rounding of the f64 value
+    ref = 0.5 * x.astype(np.float64) + 0.75 * dx.astype(np.float64)
+    np.testing.assert_allclose(r["x"], ref, rtol=0, atol=4e-8)
+    assert hist.sum() == x.size  # every ray lands inside the 128x128 detector
+    # brute force over the same pixel indices
+    Tinv = O.metres_to_pixels_matrix(**O.PLAIN_DET)
+    py, px = O.metres_to_pixels(r["x"], r["y"], Tinv)
+    np.testing.assert_array_equal(hist, O.into_image_loop(py, px, (128, 128)))
+
+
+def test_planted_rounding_mistake_is_caught():
+    # floor instead of half-even rounding must break the known-answer table
+    Tinv = O.metres_to_pixels_matrix((0.0, 0.0), (0.1, 0.1), (11, 11))
+    py, px = O.transform(0.5, -0.5, Tinv)  # (x, y) = (-0.5, 0.5)? no: y=0.5, x=-0.5
+    good = (int(O.round_to_i32(py)), int(O.round_to_i32(px)))
+    bad = (int(np.floor(py - 1e-9)), int(np.floor(px - 1e-9)))
+    assert good == (0, 0) and bad != good
