@@ -172,7 +172,9 @@ def test_broadcast_mixed_precision(device):
     assert tuple(out.x.shape) == (S, J) and out.x.dtype == torch.float64
     np.testing.assert_array_equal(_host(out.x), ref["x"])
     np.testing.assert_array_equal(_host(out.y), ref["y"])
-    assert tuple(out.dx.shape) == (1, J)
+    # the descan slope term depends on the [S,1] scan positions as well
+    assert tuple(out.dx.shape) == (S, J) == ref["dx"].shape
+    np.testing.assert_array_equal(_host(out.dx), ref["dx"])
 
 
 def test_user_defined_component(device):
