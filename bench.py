@@ -4,8 +4,13 @@
 
 Prints ONE JSON line (rank 0).  Workloads:
 
-  plain  1,048,576 f32 rays through Lens(0.5, f=1) -> Detector(1.0, 128x128):
-         fused run_to_image (positions, pathlength and int32 histogram).
+  stem_fit  (default) 4D-STEM descan/geometry fit, forward + adjoint: 64x64 scan
+            positions x 4096 rays x 8 camera lengths (134,217,728 ray evaluations)
+            against 1.07 GB of f32 target coordinates; loss + 26 gradients.
+  stem_fwd  4D-STEM forward: 16.7M rays -> detector coords (f32) + bilinear
+            sample values.
+  plain     1,048,576 f32 rays through Lens(0.5, f=1) -> Detector(1.0, 128x128):
+            fused run_to_image (positions, pathlength and int32 histogram).
 
 A "step" is one pass of the hot path over one batch whose inputs are already
 resident in HBM.  Timing: W untimed warm-up steps, then K steps bracketed by
@@ -88,8 +93,145 @@ def bench_plain(steps: int, warmup: int):
     )
 
 
-WORKLOADS = {"plain": bench_plain}
-DEFAULT = "plain"
+def _cpu_info():
+    cores = len(os.sched_getaffinity(0))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
+def _time_graph(step_fn, steps, warmup):
+    """Capture one step in a HIP graph; time `steps` replays with events."""
+    import torch
+
+    for _ in range(max(warmup, 1)):
+        step_fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        step_fn()
+    torch.cuda.current_stream().wait_stream(s)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step_fn()
+    for _ in range(warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    ev = _events(2 * steps)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ev[2 * i].record()
+        graph.replay()
+        ev[2 * i + 1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = sum(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(steps)) / steps
+    return wall * 1e3 / steps, kern_ms
+
+
+def bench_stem_fit(steps: int, warmup: int):
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from oracle import c as cport
+    from temgymcore_amd import stem
+
+    dev = torch.device("cuda", 0)
+    model, _ = stem.reference_workload(n_cam=8)
+    params = stem.params_tensor(model, dev)
+    targets = stem.stem_forward(stem.perturbed(model))["coords"]
+    ws = stem.STEMFitWorkspace(model, dev)
+    units = model.n_cam * model.n_scan * model.n_rays
+    ms_per_step, kern_ms = _time_graph(lambda: ws.run(params, targets), steps, warmup)
+    gpu_out = ws.run(params, targets).cpu().numpy()
+    bytes_per_launch = 8 * units  # f32 (x, y) target per ray and camera length, read once
+    achieved = bytes_per_launch / (kern_ms * 1e-3)
+
+    # CPU baseline: the OpenMP C port on a bounded sample of scan positions
+    lib = cport.load()
+    cores, cpu_model = _cpu_info()
+    t_host = targets.cpu().numpy()
+    off = np.zeros(4)
+    p_host = model.params()
+    out = np.zeros(ws.np)
+    n_s, elapsed, reps = 32, 0.0, 0
+    while elapsed < 10.0:
+        c0 = time.perf_counter()
+        lib.stem_fit(64, 64, 64, 8, model.semi_conv, model.defocus, model.pivot, off.ctypes.data,
+                     p_host.ctypes.data, t_host.ctypes.data, 0, n_s, out.ctypes.data)
+        elapsed += time.perf_counter() - c0
+        reps += 1
+        if elapsed < 1.0 and n_s < model.n_scan:
+            n_s, elapsed, reps = min(model.n_scan, n_s * 4), 0.0, 0
+    cpu_rate = reps * n_s * model.n_rays * model.n_cam / elapsed
+    # parity of the timed run on the same sample: GPU partial rows vs the CPU port
+    rows = ws.partials.cpu().numpy()[:n_s].sum(axis=0)
+    rel = float(np.max(np.abs(rows - out) / np.maximum(np.abs(out), 1e-300)))
+    return dict(
+        metric="rays/s fwd+bwd (16.7M-ray 4D-STEM model x 8 camera lengths: descan/geometry fit loss + 26 grads)",
+        value=units / (ms_per_step * 1e-3), unit="ray-evals/s", ms_per_step=ms_per_step, dtype="f32 (f64 reductions)",
+        config={"workload": "stem_fit: 64x64 scan, 64x64 rays, 8 camera lengths, 1.07 GB f32 targets",
+                "units_per_step": units, "graph_replay": True, "event_ms_per_step": kern_ms,
+                "cpu_port_rel_diff_on_sample": rel, "loss": float(gpu_out[-1])},
+        roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK, "traffic": None},
+        cpu_baseline={"value": cpu_rate, "unit": "ray-evals/s", "cores": cores, "kind": "port",
+                      "sample": f"OpenMP C port, {n_s} scan positions x 4096 rays x 8 camera lengths, "
+                                f"{reps} reps ({cpu_model})"},
+    )
+
+
+def bench_stem_fwd(steps: int, warmup: int):
+    import numpy as np
+    import torch
+
+    from oracle import stem_oracle as SO
+    from temgymcore_amd import stem
+
+    dev = torch.device("cuda", 0)
+    model, sample = stem.reference_workload(n_cam=1)
+    params = stem.params_tensor(model, dev)
+    sample_t = torch.from_numpy(sample).to(dev)
+    out = stem.stem_forward(model, params, sample_t, coords=True, values=True)
+    units = model.n_scan * model.n_rays
+    ms_per_step, kern_ms = _time_graph(
+        lambda: stem.stem_forward(model, params, sample_t, coords=True, values=True, out=out), steps, warmup)
+    bytes_per_launch = 12 * units + sample.nbytes  # (x, y, value) f32 written per ray + image read once
+    achieved = bytes_per_launch / (kern_ms * 1e-3)
+    geom = dict(scan_shape=model.scan_shape, n_side=model.n_side, semi_conv=model.semi_conv,
+                defocus=model.defocus, pivot=model.pivot, offsets=model.descan_offsets)
+    cores, cpu_model = _cpu_info()
+    idx = np.arange(16)
+    reps, c0 = 0, time.perf_counter()
+    while time.perf_counter() - c0 < 10.0:
+        SO.forward(geom, model.params(), scan_idx=idx)
+        SO.sample_plane(geom, model.params())
+        reps += 1
+    cpu_s = (time.perf_counter() - c0) / reps
+    return dict(
+        metric="rays/s (4D-STEM shadow-image forward: 16.7M rays -> detector coords + bilinear sample values)",
+        value=units / (ms_per_step * 1e-3), unit="rays/s", ms_per_step=ms_per_step, dtype="f32",
+        config={"workload": "stem_fwd: 64x64 scan, 64x64 rays, camera length 0.5 m", "rays": units,
+                "event_ms_per_step": kern_ms},
+        roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK, "traffic": None},
+        cpu_baseline={"value": 16 * model.n_rays / cpu_s, "unit": "rays/s", "cores": 1, "kind": "port",
+                      "sample": f"numpy f64 oracle, 16 scan positions x 4096 rays ({cpu_model})"},
+    )
+
+
+WORKLOADS = {"stem_fit": bench_stem_fit, "stem_fwd": bench_stem_fwd, "plain": bench_plain}
+DEFAULT = "stem_fit"
 
 
 def main():

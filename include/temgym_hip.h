@@ -129,6 +129,61 @@ int tg_ray_program_run(const tg_program* prog, const tg_bin_spec* bin, void* str
 /* Number of compute units of the current device (cached per device). */
 int32_t tg_device_cu_count(void);
 
+/* ======================================================================== *
+ * 4D-STEM shadow-image model (scan x camera-length batches)
+ *
+ * Per scan position s and ray j the model is the component chain
+ *   point source at z=0 (x = y = 0, slopes on an n_side x n_side grid)
+ *   -> Scanner(z=0, sp, st)  -> sample plane at z=defocus
+ *   -> descanner with a 4x4 error map of (spx, spy, stx, sty) plus offsets
+ *   -> Detector at z = defocus + L_c,
+ * with scan positions from a rotated ScanGrid (reference components.py:218-372,
+ * coordinate_transforms.py:50-101) and scan tilt st = pivot * sp.
+ * The descanner reduces to the reference Descanner (components.py:320-372)
+ * when the tilt columns of the 4x4 map are zero.  Detector coordinates are
+ * affine in the ray slopes for every (s, c), which the kernels exploit.
+ *
+ * Parameter vector (device, f64, 18 + n_cam entries):
+ *   [0..15]  descan map A, row-major: rows (x, y, dx, dy) of the output
+ *            offset, columns (spx, spy, stx, sty)
+ *   [16]     scan step (m)      [17] scan rotation (degrees)
+ *   [18+c]   camera length L_c (m)
+ * ======================================================================== */
+
+#define TG_STEM_MAX_CAM 16
+
+typedef struct tg_stem_geometry {
+  int32_t scan_h, scan_w;   /* scan grid shape (rows, cols)                    */
+  int32_t n_side;           /* rays per side; J = n_side^2 (n_side even)      */
+  int32_t n_cam;            /* camera lengths (1..TG_STEM_MAX_CAM)            */
+  double semi_conv;         /* slopes (k - (n_side-1)/2) * 2*semi_conv/n_side  */
+  double defocus;           /* source -> sample plane distance (m)            */
+  double pivot;             /* scan tilt per metre of scan position (rad/m)    */
+  double offsets[4];        /* descan offsets added to x, y, dx, dy           */
+  int32_t img_h, img_w;     /* sample image shape                              */
+  double img_m[6];          /* metres -> fractional sample pixel (py, px)      */
+  int32_t det_h, det_w;     /* detector shape                                  */
+  double det_m[6];          /* metres -> detector pixel (py, px)               */
+} tg_stem_geometry;
+
+/* Forward model for every (camera length, scan position, ray).
+ *   coords  [n_cam, S, J, 2] f32 detector (x, y) or NULL
+ *   values  [S, J] f32 bilinear sample-image value at the sample plane or NULL
+ *   image4d [S, det_h, det_w] f32 (accumulated, camera length 0) or NULL:
+ *           each ray adds its sample value at its detector pixel (binning
+ *           semantics of Grid.into_image).  The caller zeroes it.
+ * `image` is the [img_h, img_w] f32 sample; required if values or image4d. */
+int tg_stem_forward(const tg_stem_geometry* g, const double* params, const float* image, float* coords,
+                    float* values, float* image4d, void* stream);
+
+/* Descan / geometry fit, forward and adjoint in one pass over the targets.
+ *   targets  [n_cam, S, J, 2] f32 target detector coordinates
+ *   partials [S, 18 + n_cam + 1] f64 workspace (per-scan-position terms)
+ *   out      [18 + n_cam + 1] f64: d loss / d params, then the loss
+ * loss = sum over (c, s, j) of |detector(x, y) - target|^2. */
+int tg_stem_fit(const tg_stem_geometry* g, const double* params, const float* targets, double* partials,
+                double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

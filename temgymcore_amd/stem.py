@@ -1,0 +1,316 @@
+"""4D-STEM shadow-image model on the HIP kernels (forward + descan/geometry fit).
+
+The model is the reference's component chain for one scan position
+(reference components.py:218-406, run.py:85-116):
+
+    point source (x = y = 0, slopes on an n x n grid inside the cone)
+    -> Scanner(z=0, scan_pos, scan_tilt)          scan tilt = pivot * scan_pos
+    -> ScanGrid(z=defocus)                        the sample plane
+    -> TiltDescanner(z=defocus, 4x4 error map)    == Descanner for zero tilt columns
+    -> Detector(z=defocus + L_c)
+
+``STEMModel.components(c)`` returns exactly that list (scan positions as
+[S, 1] tensors, ray slopes as [1, J] tensors) for the generic ``run_to_end``
+path; ``stem_forward`` and ``descan_fit_loss`` run the specialised kernels
+that evaluate it for all scan positions and camera lengths in one launch.
+
+Parameter vector (float64, 18 + n_cam): descan map A (16, row-major; rows are
+the x, y, dx, dy offsets, columns spx, spy, stx, sty), scan step (m), scan
+rotation (degrees), camera lengths (m).
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+import torch
+
+from . import _lib
+from .components import Component, Detector, ScanGrid, Scanner
+from .ray import Ray
+
+
+class StemGeometry(ctypes.Structure):
+    _fields_ = [
+        ("scan_h", ctypes.c_int32), ("scan_w", ctypes.c_int32),
+        ("n_side", ctypes.c_int32), ("n_cam", ctypes.c_int32),
+        ("semi_conv", ctypes.c_double), ("defocus", ctypes.c_double), ("pivot", ctypes.c_double),
+        ("offsets", ctypes.c_double * 4),
+        ("img_h", ctypes.c_int32), ("img_w", ctypes.c_int32),
+        ("img_m", ctypes.c_double * 6),
+        ("det_h", ctypes.c_int32), ("det_w", ctypes.c_int32),
+        ("det_m", ctypes.c_double * 6),
+    ]
+
+
+_P = ctypes.c_void_p
+_lib.register_signatures({
+    "tg_stem_forward": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, _P, _P, _P, _P]),
+    "tg_stem_fit": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, _P, _P, _P]),
+})
+
+MAX_CAM = 16
+
+
+@dataclasses.dataclass(frozen=True)
+class TiltDescanner(Component):
+    """Descanner with a full 4x4 error map of (spx, spy, stx, sty).
+
+    x += (A[0].v + off[0] - spx), y += (A[1].v + off[1] - spy),
+    dx += (A[2].v + off[2] - stx), dy += (A[3].v + off[3] - sty), v = (spx, spy, stx, sty).
+    With the tilt columns of A zero this is the reference Descanner
+    (components.py:320-372) with DescanError(pxo_pxi=A[0][0], pxo_pyi=A[0][1], ...).
+    """
+
+    z: float
+    scan_pos_x: object
+    scan_pos_y: object
+    scan_tilt_x: object = 0.0
+    scan_tilt_y: object = 0.0
+    descan_matrix: tuple = ((0.0,) * 4,) * 4
+    descan_offsets: tuple = (0.0, 0.0, 0.0, 0.0)
+
+    def __call__(self, ray: Ray):
+        A, o = self.descan_matrix, self.descan_offsets
+        spx, spy, stx, sty = self.scan_pos_x, self.scan_pos_y, self.scan_tilt_x, self.scan_tilt_y
+
+        def row(r):
+            return spx * A[r][0] + spy * A[r][1] + stx * A[r][2] + sty * A[r][3] + o[r]
+
+        return ray.derive(
+            x=ray.x + (row(0) - spx) * ray._one,
+            y=ray.y + (row(1) - spy) * ray._one,
+            dx=ray.dx + (row(2) - stx) * ray._one,
+            dy=ray.dy + (row(3) - sty) * ray._one,
+        )
+
+
+@dataclasses.dataclass
+class STEMModel:
+    """Geometry and parameters of the 4D-STEM shadow-image model."""
+
+    scan_shape: tuple = (64, 64)
+    scan_step: float = 1e-10
+    scan_rotation: float = 12.0
+    n_side: int = 64
+    semi_conv: float = 0.5e-3
+    defocus: float = 200e-9
+    pivot: float = 1e4
+    descan: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros((4, 4)))
+    descan_offsets: tuple = (0.0, 0.0, 0.0, 0.0)
+    camera_lengths: tuple = (0.5,)
+    det_shape: tuple = (256, 256)
+    det_pixel: float = 2.5e-6
+    sample_shape: tuple = (512, 512)
+    sample_pixel: float = 2e-11
+
+    # -- derived ---------------------------------------------------------------
+    @property
+    def n_scan(self) -> int:
+        return int(self.scan_shape[0] * self.scan_shape[1])
+
+    @property
+    def n_rays(self) -> int:
+        return int(self.n_side * self.n_side)
+
+    @property
+    def n_cam(self) -> int:
+        return len(self.camera_lengths)
+
+    def params(self) -> np.ndarray:
+        """The float64 parameter vector (descan 16, step, rotation, camera lengths)."""
+        return np.concatenate([np.asarray(self.descan, dtype=np.float64).reshape(16),
+                               [self.scan_step, self.scan_rotation],
+                               np.asarray(self.camera_lengths, dtype=np.float64)])
+
+    def with_params(self, p) -> "STEMModel":
+        p = np.asarray(p, dtype=np.float64)
+        return dataclasses.replace(self, descan=p[:16].reshape(4, 4).copy(), scan_step=float(p[16]),
+                                   scan_rotation=float(p[17]), camera_lengths=tuple(float(v) for v in p[18:]))
+
+    def scan_grid(self) -> ScanGrid:
+        return ScanGrid(z=self.defocus, pixel_size=(self.scan_step, self.scan_step), shape=tuple(self.scan_shape),
+                        rotation=self.scan_rotation)
+
+    def sample_grid(self) -> Detector:
+        return Detector(z=self.defocus, pixel_size=(self.sample_pixel, self.sample_pixel),
+                        shape=tuple(self.sample_shape))
+
+    def detector(self, c: int = 0) -> Detector:
+        return Detector(z=self.defocus + self.camera_lengths[c], pixel_size=(self.det_pixel, self.det_pixel),
+                        shape=tuple(self.det_shape))
+
+    def slopes(self) -> np.ndarray:
+        """f32 slopes of one grid axis: (k - (n-1)/2) * 2*semi_conv/n."""
+        k = np.arange(self.n_side, dtype=np.float64)
+        return ((k - 0.5 * (self.n_side - 1)) * (2.0 * self.semi_conv / self.n_side)).astype(np.float32)
+
+    def scan_positions(self) -> tuple[np.ndarray, np.ndarray]:
+        """(spx, spy) f64 for flattened scan index s = row * scan_w + col."""
+        yy, xx = np.meshgrid(np.arange(self.scan_shape[0]), np.arange(self.scan_shape[1]), indexing="ij")
+        spx, spy = self.scan_grid().pixels_to_metres((yy.ravel(), xx.ravel()))
+        return np.asarray(spx, dtype=np.float64), np.asarray(spy, dtype=np.float64)
+
+    def geometry(self) -> StemGeometry:
+        if not 1 <= self.n_cam <= MAX_CAM:
+            raise ValueError(f"1..{MAX_CAM} camera lengths supported, got {self.n_cam}")
+        if self.n_side % 2:
+            raise ValueError("n_side must be even")
+        g = StemGeometry()
+        g.scan_h, g.scan_w = int(self.scan_shape[0]), int(self.scan_shape[1])
+        g.n_side, g.n_cam = int(self.n_side), int(self.n_cam)
+        g.semi_conv, g.defocus, g.pivot = float(self.semi_conv), float(self.defocus), float(self.pivot)
+        for k in range(4):
+            g.offsets[k] = float(self.descan_offsets[k])
+        g.img_h, g.img_w = int(self.sample_shape[0]), int(self.sample_shape[1])
+        M = self.sample_grid().metres_to_pixels_mat
+        for k, v in enumerate((M[0, 0], M[0, 1], M[0, 2], M[1, 0], M[1, 1], M[1, 2])):
+            g.img_m[k] = float(v)
+        g.det_h, g.det_w = int(self.det_shape[0]), int(self.det_shape[1])
+        M = self.detector(0).metres_to_pixels_mat
+        for k, v in enumerate((M[0, 0], M[0, 1], M[0, 2], M[1, 0], M[1, 1], M[1, 2])):
+            g.det_m[k] = float(v)
+        return g
+
+    # -- the same model as a reference component list -----------------------------
+    def rays(self, device=None) -> Ray:
+        """Point-source rays: x = y = 0, slopes [1, J] (row-major over (dy, dx))."""
+        device = _lib.require_device() if device is None else device
+        t = self.slopes()
+        dy, dx = np.meshgrid(t, t, indexing="ij")
+        return Ray(x=0.0, y=0.0, dx=torch.from_numpy(dx.reshape(1, -1).copy()).to(device),
+                   dy=torch.from_numpy(dy.reshape(1, -1).copy()).to(device), z=0.0, pathlength=0.0)
+
+    def components(self, c: int = 0, device=None) -> list:
+        device = _lib.require_device() if device is None else device
+        spx, spy = self.scan_positions()
+        spx_t = torch.from_numpy(spx.reshape(-1, 1)).to(device)
+        spy_t = torch.from_numpy(spy.reshape(-1, 1)).to(device)
+        stx_t = torch.from_numpy((self.pivot * spx).reshape(-1, 1)).to(device)
+        sty_t = torch.from_numpy((self.pivot * spy).reshape(-1, 1)).to(device)
+        A = tuple(tuple(float(v) for v in row) for row in np.asarray(self.descan, dtype=np.float64))
+        return [
+            Scanner(z=0.0, scan_pos_x=spx_t, scan_pos_y=spy_t, scan_tilt_x=stx_t, scan_tilt_y=sty_t),
+            self.scan_grid(),
+            TiltDescanner(z=self.defocus, scan_pos_x=spx_t, scan_pos_y=spy_t, scan_tilt_x=stx_t,
+                          scan_tilt_y=sty_t, descan_matrix=A, descan_offsets=tuple(self.descan_offsets)),
+            self.detector(c),
+        ]
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def params_tensor(model: STEMModel, device=None) -> torch.Tensor:
+    device = _lib.require_device() if device is None else device
+    return torch.from_numpy(model.params()).to(device)
+
+
+def stem_forward(model: STEMModel, params: torch.Tensor | None = None, sample: torch.Tensor | None = None,
+                 coords: bool = True, values: bool = False, image4d: bool = False, out=None):
+    """Run the forward model for all (camera length, scan position, ray).
+
+    Returns a dict with `coords` [n_cam, S, J, 2] f32, `values` [S, J] f32 and
+    `image4d` [S, det_h, det_w] f32 (camera length 0) as requested.
+    """
+    device = _lib.require_device()
+    lib = _lib.load()
+    params = params_tensor(model, device) if params is None else params
+    _check_params(model, params)
+    S, J, C = model.n_scan, model.n_rays, model.n_cam
+    res = {} if out is None else out
+    if coords and "coords" not in res:
+        res["coords"] = torch.empty((C, S, J, 2), dtype=torch.float32, device=device)
+    if values and "values" not in res:
+        res["values"] = torch.empty((S, J), dtype=torch.float32, device=device)
+    if image4d and "image4d" not in res:
+        res["image4d"] = torch.zeros((S,) + tuple(model.det_shape), dtype=torch.float32, device=device)
+    if (values or image4d) and sample is None:
+        raise ValueError("a sample image is needed for values / image4d")
+    if sample is not None:
+        if tuple(sample.shape) != tuple(model.sample_shape) or sample.dtype != torch.float32 or not sample.is_cuda:
+            raise ValueError(f"sample must be a device float32 tensor of shape {tuple(model.sample_shape)}")
+        sample = sample.contiguous()
+    g = model.geometry()
+    st = lib.tg_stem_forward(ctypes.byref(g), params.data_ptr(), _ptr(sample), _ptr(res.get("coords") if coords else None),
+                             _ptr(res.get("values") if values else None),
+                             _ptr(res.get("image4d") if image4d else None), _lib.stream_ptr(device))
+    _lib.check(st, "tg_stem_forward")
+    return res
+
+
+def _check_params(model, params):
+    if not (isinstance(params, torch.Tensor) and params.is_cuda and params.dtype == torch.float64
+            and params.is_contiguous() and params.numel() == 18 + model.n_cam):
+        raise ValueError(f"params must be a contiguous device float64 tensor of {18 + model.n_cam} entries")
+
+
+class STEMFitWorkspace:
+    """Device buffers of one fit configuration (reused across calls and graph replays)."""
+
+    def __init__(self, model: STEMModel, device=None):
+        device = _lib.require_device() if device is None else device
+        self.model = model
+        self.geometry = model.geometry()
+        self.np = 18 + model.n_cam + 1
+        self.partials = torch.empty((model.n_scan, self.np), dtype=torch.float64, device=device)
+        self.out = torch.empty(self.np, dtype=torch.float64, device=device)
+
+    def run(self, params: torch.Tensor, targets: torch.Tensor) -> torch.Tensor:
+        """Enqueue the fused forward+adjoint; returns the [grad..., loss] vector."""
+        m = self.model
+        _check_params(m, params)
+        shape = (m.n_cam, m.n_scan, m.n_rays, 2)
+        if tuple(targets.shape) != shape or targets.dtype != torch.float32 or not targets.is_cuda \
+                or not targets.is_contiguous():
+            raise ValueError(f"targets must be a contiguous device float32 tensor of shape {shape}")
+        st = _lib.load().tg_stem_fit(ctypes.byref(self.geometry), params.data_ptr(), targets.data_ptr(),
+                                     self.partials.data_ptr(), self.out.data_ptr(),
+                                     _lib.stream_ptr(targets.device))
+        _lib.check(st, "tg_stem_fit")
+        return self.out
+
+
+class _DescanFitLoss(torch.autograd.Function):
+    """loss = sum |detector(params) - targets|^2 with the hand-written adjoint.
+
+    The kernel computes loss and gradient in one pass over the targets; the
+    backward multiplies the stored gradient by the incoming cotangent."""
+
+    @staticmethod
+    def forward(ctx, params, targets, ws: STEMFitWorkspace):
+        out = ws.run(params.detach(), targets).clone()
+        ctx.save_for_backward(out[:-1])
+        return out[-1]
+
+    @staticmethod
+    def backward(ctx, grad_loss):
+        (g,) = ctx.saved_tensors
+        return grad_loss * g, None, None
+
+
+def descan_fit_loss(model: STEMModel, params: torch.Tensor, targets: torch.Tensor,
+                    workspace: STEMFitWorkspace | None = None) -> torch.Tensor:
+    """Differentiable L2 descan/geometry-fit loss (scalar tensor)."""
+    ws = STEMFitWorkspace(model, params.device) if workspace is None else workspace
+    return _DescanFitLoss.apply(params, targets, ws)
+
+
+def reference_workload(n_cam: int = 1, seed_descan: int = 1, seed_sample: int = 2, **overrides):
+    """The seeded 4D-STEM configuration: 64x64 scan (0.1 nm, 12 deg), 64x64 rays
+    in a 0.5 mrad cone, defocus 200 nm, descan map ~ N(0, 1e-3), 256^2 detector
+    (2.5 um pixels), 512^2 random-phase sample (0.02 nm pixels).  With
+    n_cam == 1 the camera length is 0.5 m, else linspace(0.1, 1.0, n_cam)."""
+    cams = (0.5,) if n_cam == 1 else tuple(np.linspace(0.1, 1.0, n_cam))
+    descan = np.random.default_rng(seed_descan).normal(0.0, 1e-3, (4, 4))
+    model = STEMModel(descan=descan, camera_lengths=cams, **overrides)
+    sample = np.random.default_rng(seed_sample).uniform(-np.pi, np.pi, model.sample_shape).astype(np.float32)
+    return model, sample
+
+
+def perturbed(model: STEMModel, seed: int = 3, rel: float = 1e-3) -> STEMModel:
+    """Every parameter scaled by (1 + rel * N(0, 1)) (target generation)."""
+    p = model.params()
+    return model.with_params(p * (1.0 + rel * np.random.default_rng(seed).normal(size=p.shape)))

@@ -1,0 +1,170 @@
+"""GPU parity of the 4D-STEM kernels against the float64 oracle.
+
+Tolerances (stated per the north-star precision policy):
+  detector coordinates: |d| <= 1e-6 * max(|ref|, 1e-6 m)   (f32 output)
+  sample values:        |d| <= 2e-5 (values in [-pi, pi], f32 bilinear weights)
+  fit gradient:         |d| <= 2e-4 * max|g| per block of parameters, vs central
+                        FD of the float64 loss on the same f32 targets
+  loss:                 relative 1e-5
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ray_oracle as R
+from oracle import stem_oracle as SO
+from temgymcore_amd import stem
+from temgymcore_amd.run import run_to_end
+
+pytestmark = pytest.mark.gpu
+
+
+def _geom(m):
+    return dict(scan_shape=m.scan_shape, n_side=m.n_side, semi_conv=m.semi_conv, defocus=m.defocus,
+                pivot=m.pivot, offsets=tuple(m.descan_offsets))
+
+
+def _small(n_cam=3, seed=5, **kw):
+    model, sample = stem.reference_workload(n_cam=n_cam, **kw)
+    return model, sample
+
+
+def _close_pos(got, ref, floor=1e-6):
+    bound = 1e-6 * np.maximum(np.abs(ref), floor)
+    err = np.abs(got.astype(np.float64) - ref)
+    assert np.all(err <= bound), f"max err {err.max():.3e}, worst ratio {(err / bound).max():.3f}"
+
+
+@pytest.mark.parametrize("shape", [(4, 6), (64, 64)])
+def test_forward_coords_vs_oracle(device, shape):
+    print(f"[stem forward scan={shape}]", flush=True)
+    model, _ = stem.reference_workload(n_cam=8, scan_shape=shape)
+    out = stem.stem_forward(model)["coords"].cpu().numpy()
+    x, y = SO.forward(_geom(model), model.params())
+    _close_pos(out[..., 0], x)
+    _close_pos(out[..., 1], y)
+
+
+def test_forward_full_size_single_camera(device):
+    print("[stem forward full 16.7M rays]", flush=True)
+    model, sample = stem.reference_workload(n_cam=1)
+    res = stem.stem_forward(model, sample=torch.from_numpy(sample).to(device), coords=True, values=True)
+    coords = res["coords"].cpu().numpy()
+    x, y = SO.forward(_geom(model), model.params())
+    _close_pos(coords[..., 0], x)
+    _close_pos(coords[..., 1], y)
+    xs, ys = SO.sample_plane(_geom(model), model.params())
+    Tinv = R.metres_to_pixels_matrix((0.0, 0.0), (model.sample_pixel,) * 2, model.sample_shape)
+    ref_v = SO.bilinear(sample.astype(np.float64), Tinv, xs, ys)
+    np.testing.assert_allclose(res["values"].cpu().numpy(), ref_v, rtol=0, atol=2e-5)
+
+
+def test_image4d_deposit(device):
+    model, sample = stem.reference_workload(n_cam=1, scan_shape=(8, 8))
+    res = stem.stem_forward(model, sample=torch.from_numpy(sample).to(device), coords=True, values=True,
+                            image4d=True)
+    img = res["image4d"].cpu().numpy().astype(np.float64)
+    coords = res["coords"].cpu().numpy()[0]
+    vals = res["values"].cpu().numpy().astype(np.float64)
+    Tinv = R.metres_to_pixels_matrix((0.0, 0.0), (model.det_pixel,) * 2, model.det_shape)
+    ref = np.zeros_like(img)
+    for s in range(model.n_scan):
+        py, px = R.metres_to_pixels(coords[s, :, 0], coords[s, :, 1], Tinv)
+        ok = (py >= 0) & (py < model.det_shape[0]) & (px >= 0) & (px < model.det_shape[1])
+        np.add.at(ref[s], (py[ok], px[ok]), vals[s][ok])
+    np.testing.assert_allclose(img, ref, rtol=1e-6, atol=1e-5)
+
+
+def test_generic_component_path_matches_oracle_and_kernel(device):
+    """The same model through run_to_end (ray programs) is bit-exact vs the
+    component oracle and within tolerance of the fused STEM kernel."""
+    model, _ = stem.reference_workload(n_cam=1, scan_shape=(6, 5), n_side=16)
+    out = run_to_end(model.rays(device), model.components(0, device))
+    spx, spy = model.scan_positions()
+    spx, spy = spx.reshape(-1, 1), spy.reshape(-1, 1)
+    t = model.slopes()
+    ty, tx = np.meshgrid(t, t, indexing="ij")
+    r = R.make_ray(0.0, 0.0, tx.reshape(1, -1), ty.reshape(1, -1))
+    r = R.scanner(r, spx, spy, model.pivot * spx, model.pivot * spy)
+    r = R.propagate(r, model.defocus - r["z"])
+    r = SO.descanner4(r, spx, spy, model.pivot * spx, model.pivot * spy, model.descan, model.descan_offsets)
+    r = R.propagate(r, model.defocus + model.camera_lengths[0] - r["z"])
+    np.testing.assert_array_equal(out.x.cpu().numpy(), r["x"])
+    np.testing.assert_array_equal(out.y.cpu().numpy(), r["y"])
+    k = stem.stem_forward(model)["coords"].cpu().numpy()[0]
+    _close_pos(k[..., 0], out.x.cpu().numpy())
+    _close_pos(k[..., 1], out.y.cpu().numpy())
+
+
+def _targets(model, device):
+    tgt_model = stem.perturbed(model)
+    return stem.stem_forward(tgt_model)["coords"]
+
+
+def test_fit_gradient_vs_fd_small(device):
+    model, _ = stem.reference_workload(n_cam=3, scan_shape=(6, 7), n_side=16)
+    targets = _targets(model, device)
+    ws = stem.STEMFitWorkspace(model, device)
+    out = ws.run(stem.params_tensor(model, device), targets).cpu().numpy()
+    t = targets.cpu().numpy()
+    ref_loss = SO.loss(_geom(model), model.params(), t)
+    ref_g = SO.fd_grad(_geom(model), model.params(), t)
+    assert out[-1] == pytest.approx(ref_loss, rel=1e-5)
+    g = out[:-1]
+    for blk in (slice(0, 16), slice(16, 17), slice(17, 18), slice(18, None)):
+        scale = np.abs(ref_g[blk]).max()
+        np.testing.assert_allclose(g[blk], ref_g[blk], rtol=0, atol=2e-4 * scale, err_msg=str(blk))
+
+
+def test_fit_full_size_properties(device):
+    """Full 64x64 scan x 4096 rays x 8 camera lengths: zero loss/gradient at the
+    generating parameters, and per-scan-position gradient rows that match the
+    FD oracle on a sample of scan positions."""
+    print("[stem fit full size]", flush=True)
+    model, _ = stem.reference_workload(n_cam=8)
+    params = stem.params_tensor(model, device)
+    ws = stem.STEMFitWorkspace(model, device)
+    exact = stem.stem_forward(model)["coords"]
+    out0 = ws.run(params, exact).cpu().numpy()
+    # targets from the same f32 forward: residuals are the f32 rounding only
+    assert abs(out0[-1]) < 1e-16 * model.n_scan * model.n_rays * model.n_cam
+    targets = _targets(model, device)
+    out = ws.run(params, targets).cpu().numpy()
+    rows = ws.partials.cpu().numpy()
+    np.testing.assert_allclose(rows.sum(axis=0), out, rtol=1e-12, atol=0)
+    pick = np.random.default_rng(0).choice(model.n_scan, 6, replace=False)
+    t = targets.cpu().numpy()[:, pick]
+    ref_g = SO.fd_grad(_geom(model), model.params(), t, scan_idx=pick)
+    ref_loss = SO.loss(_geom(model), model.params(), t, scan_idx=pick)
+    sub = rows[pick].sum(axis=0)
+    assert sub[-1] == pytest.approx(ref_loss, rel=1e-5)
+    for blk in (slice(0, 16), slice(16, 17), slice(17, 18), slice(18, 26)):
+        scale = np.abs(ref_g[blk]).max()
+        np.testing.assert_allclose(sub[blk], ref_g[blk], rtol=0, atol=2e-4 * scale, err_msg=str(blk))
+
+
+def test_autograd_function_and_graph_replay(device):
+    model, _ = stem.reference_workload(n_cam=2, scan_shape=(16, 16))
+    targets = _targets(model, device)
+    params = stem.params_tensor(model, device).requires_grad_(True)
+    ws = stem.STEMFitWorkspace(model, device)
+    loss = stem.descan_fit_loss(model, params, targets, ws)
+    (3.0 * loss).backward()
+    eager = ws.run(params.detach(), targets).clone()
+    torch.testing.assert_close(params.grad, 3.0 * eager[:-1], rtol=0, atol=0)
+    # HIP graph capture of the fused fwd+bwd; replays match eager every time
+    p2 = params.detach().clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        ws.run(p2, targets)
+    torch.cuda.current_stream().wait_stream(s)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        ws.run(p2, targets)
+    for k in range(3):
+        ws.out.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        torch.testing.assert_close(ws.out, eager, rtol=0, atol=0)
+        _ = (targets[0, 0, 0] * 1.0).item()  # eager work between replays
