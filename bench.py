@@ -31,6 +31,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # B/s, MI355X
+SKIP_CPU = False
 
 
 def _events(n):
@@ -77,10 +78,10 @@ def bench_plain(steps: int, warmup: int):
 
     # CPU baseline: the numpy oracle (single thread) on the full batch, ~10 s
     reps, c0 = 0, time.perf_counter()
-    while time.perf_counter() - c0 < 10.0:
+    while not SKIP_CPU and time.perf_counter() - c0 < 10.0:
         O.plain_run(x, y, dx, dy)
         reps += 1
-    cpu_s = (time.perf_counter() - c0) / reps
+    cpu_s = (time.perf_counter() - c0) / max(reps, 1)
     return dict(
         metric="rays/s (plain run_to_image: 1,048,576 f32 rays, Lens + Detector 128x128 binning)",
         value=n / (ms_per_step * 1e-3), unit="rays/s", ms_per_step=ms_per_step, dtype="f32",
@@ -88,8 +89,8 @@ def bench_plain(steps: int, warmup: int):
                 "kernel_ms": kern_ms, "parity_x_bit_exact": parity},
         roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                   "frac": achieved / HBM_PEAK, "traffic": None},
-        cpu_baseline={"value": n / cpu_s, "unit": "rays/s", "cores": 1, "kind": "port",
-                      "sample": f"numpy oracle, full {n}-ray batch x {reps} reps"},
+        cpu_baseline=None if SKIP_CPU else {"value": n / cpu_s, "unit": "rays/s", "cores": 1, "kind": "port",
+                                            "sample": f"numpy oracle, full {n}-ray batch x {reps} reps"},
     )
 
 
@@ -107,13 +108,61 @@ def _cpu_info():
     return cores, model
 
 
-def _time_graph(step_fn, steps, warmup):
-    """Capture one step in a HIP graph; time `steps` replays with events."""
+def hbm_read_probe(n_bytes: int = 2 << 30, reps: int = 20):
+    """Achievable HBM read rate (B/s) of this device, measured with the
+    library's streaming probe kernel over a 2 GiB buffer."""
+    import ctypes
+
+    import torch
+
+    from temgymcore_amd import _lib
+
+    lib = _lib.load()
+    dev = torch.device("cuda", 0)
+    buf = torch.ones(n_bytes // 4, dtype=torch.float32, device=dev)
+    n_blocks = 8 * lib.tg_device_cu_count()
+    sums = torch.empty(n_blocks, dtype=torch.float32, device=dev)
+    stream = _lib.stream_ptr(dev)
+
+    def run():
+        _lib.check(lib.tg_hbm_read_probe(buf.data_ptr(), n_bytes, sums.data_ptr(), n_blocks, stream),
+                   "tg_hbm_read_probe")
+
+    for _ in range(3):
+        run()
+    torch.cuda.synchronize()
+    ev = _events(2 * reps)
+    for i in range(reps):
+        ev[2 * i].record()
+        run()
+        ev[2 * i + 1].record()
+    torch.cuda.synchronize()
+    ms = sorted(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(reps))[reps // 2]
+    ok = abs(float(sums.double().sum()) - n_bytes / 4) < 1e-6 * n_bytes
+    del buf
+    return n_bytes / (ms * 1e-3), ok
+
+
+def _time_steps(step_fn, steps, warmup):
+    """Time `steps` back-to-back steps on the current stream.
+
+    Returns (wall ms/step, GPU ms/step from one HIP event pair around the K
+    steps, GPU ms/step of the same K steps as HIP-graph replays)."""
     import torch
 
     for _ in range(max(warmup, 1)):
         step_fn()
     torch.cuda.synchronize()
+    e0, e1 = _events(2)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        step_fn()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    gpu = e0.elapsed_time(e1) / steps
+    # the same step captured once in a HIP graph and replayed K times
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(s):
@@ -122,19 +171,16 @@ def _time_graph(step_fn, steps, warmup):
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
         step_fn()
-    for _ in range(warmup):
+    for _ in range(max(warmup, 1)):
         graph.replay()
     torch.cuda.synchronize()
-    ev = _events(2 * steps)
-    t0 = time.perf_counter()
-    for i in range(steps):
-        ev[2 * i].record()
+    g0, g1 = _events(2)
+    g0.record()
+    for _ in range(steps):
         graph.replay()
-        ev[2 * i + 1].record()
+    g1.record()
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = sum(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(steps)) / steps
-    return wall * 1e3 / steps, kern_ms
+    return wall, gpu, g0.elapsed_time(g1) / steps
 
 
 def bench_stem_fit(steps: int, warmup: int):
@@ -152,11 +198,14 @@ def bench_stem_fit(steps: int, warmup: int):
     targets = stem.stem_forward(stem.perturbed(model))["coords"]
     ws = stem.STEMFitWorkspace(model, dev)
     units = model.n_cam * model.n_scan * model.n_rays
-    ms_per_step, kern_ms = _time_graph(lambda: ws.run(params, targets), steps, warmup)
+    ms_per_step, kern_ms, graph_ms = _time_steps(lambda: ws.run(params, targets), steps, warmup)
     gpu_out = ws.run(params, targets).cpu().numpy()
     bytes_per_launch = 8 * units  # f32 (x, y) target per ray and camera length, read once
     achieved = bytes_per_launch / (kern_ms * 1e-3)
+    probe_bw, probe_ok = hbm_read_probe()
 
+    if SKIP_CPU:
+        return _result_fit(units, ms_per_step, kern_ms, graph_ms, gpu_out, achieved, probe_bw, probe_ok, None, None)
     # CPU baseline: the OpenMP C port on a bounded sample of scan positions
     lib = cport.load()
     cores, cpu_model = _cpu_info()
@@ -175,19 +224,27 @@ def bench_stem_fit(steps: int, warmup: int):
             n_s, elapsed, reps = min(model.n_scan, n_s * 4), 0.0, 0
     cpu_rate = reps * n_s * model.n_rays * model.n_cam / elapsed
     # parity of the timed run on the same sample: GPU partial rows vs the CPU port
-    rows = ws.partials.cpu().numpy()[:n_s].sum(axis=0)
+    rows = ws.partials.cpu().numpy()[:, :n_s].sum(axis=1)
     rel = float(np.max(np.abs(rows - out) / np.maximum(np.abs(out), 1e-300)))
+    cpu = {"value": cpu_rate, "unit": "ray-evals/s", "cores": cores, "kind": "port",
+           "sample": f"OpenMP C port, {n_s} scan positions x 4096 rays x 8 camera lengths, "
+                     f"{reps} reps ({cpu_model})"}
+    return _result_fit(units, ms_per_step, kern_ms, graph_ms, gpu_out, achieved, probe_bw, probe_ok, cpu, rel)
+
+
+def _result_fit(units, ms_per_step, kern_ms, graph_ms, gpu_out, achieved, probe_bw, probe_ok, cpu, rel):
     return dict(
         metric="rays/s fwd+bwd (16.7M-ray 4D-STEM model x 8 camera lengths: descan/geometry fit loss + 26 grads)",
         value=units / (ms_per_step * 1e-3), unit="ray-evals/s", ms_per_step=ms_per_step, dtype="f32 (f64 reductions)",
         config={"workload": "stem_fit: 64x64 scan, 64x64 rays, 8 camera lengths, 1.07 GB f32 targets",
-                "units_per_step": units, "graph_replay": True, "event_ms_per_step": kern_ms,
+                "units_per_step": units, "event_ms_per_step": kern_ms, "graph_replay_ms_per_step": graph_ms,
+                "kernels_per_step": "stem_fit_kernel + row_sum_kernel (event time covers both)",
                 "cpu_port_rel_diff_on_sample": rel, "loss": float(gpu_out[-1])},
         roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK, "traffic": None},
-        cpu_baseline={"value": cpu_rate, "unit": "ray-evals/s", "cores": cores, "kind": "port",
-                      "sample": f"OpenMP C port, {n_s} scan positions x 4096 rays x 8 camera lengths, "
-                                f"{reps} reps ({cpu_model})"},
+                  "frac": achieved / HBM_PEAK, "traffic": None,
+                  "measured_read_peak": probe_bw / 1e9 if probe_ok else None,
+                  "frac_of_measured": achieved / probe_bw if probe_ok else None},
+        cpu_baseline=cpu,
     )
 
 
@@ -204,7 +261,7 @@ def bench_stem_fwd(steps: int, warmup: int):
     sample_t = torch.from_numpy(sample).to(dev)
     out = stem.stem_forward(model, params, sample_t, coords=True, values=True)
     units = model.n_scan * model.n_rays
-    ms_per_step, kern_ms = _time_graph(
+    ms_per_step, kern_ms, graph_ms = _time_steps(
         lambda: stem.stem_forward(model, params, sample_t, coords=True, values=True, out=out), steps, warmup)
     bytes_per_launch = 12 * units + sample.nbytes  # (x, y, value) f32 written per ray + image read once
     achieved = bytes_per_launch / (kern_ms * 1e-3)
@@ -213,20 +270,21 @@ def bench_stem_fwd(steps: int, warmup: int):
     cores, cpu_model = _cpu_info()
     idx = np.arange(16)
     reps, c0 = 0, time.perf_counter()
-    while time.perf_counter() - c0 < 10.0:
+    while not SKIP_CPU and time.perf_counter() - c0 < 10.0:
         SO.forward(geom, model.params(), scan_idx=idx)
         SO.sample_plane(geom, model.params())
         reps += 1
-    cpu_s = (time.perf_counter() - c0) / reps
+    cpu_s = (time.perf_counter() - c0) / max(reps, 1)
     return dict(
         metric="rays/s (4D-STEM shadow-image forward: 16.7M rays -> detector coords + bilinear sample values)",
         value=units / (ms_per_step * 1e-3), unit="rays/s", ms_per_step=ms_per_step, dtype="f32",
         config={"workload": "stem_fwd: 64x64 scan, 64x64 rays, camera length 0.5 m", "rays": units,
-                "event_ms_per_step": kern_ms},
+                "event_ms_per_step": kern_ms, "graph_replay_ms_per_step": graph_ms},
         roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                   "frac": achieved / HBM_PEAK, "traffic": None},
-        cpu_baseline={"value": 16 * model.n_rays / cpu_s, "unit": "rays/s", "cores": 1, "kind": "port",
-                      "sample": f"numpy f64 oracle, 16 scan positions x 4096 rays ({cpu_model})"},
+        cpu_baseline=None if SKIP_CPU else {
+            "value": 16 * model.n_rays / cpu_s, "unit": "rays/s", "cores": 1, "kind": "port",
+            "sample": f"numpy f64 oracle, 16 scan positions x 4096 rays ({cpu_model})"},
     )
 
 
@@ -240,7 +298,11 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the CPU leg (for profiler runs; the JSON line then has cpu_baseline null)")
     args = ap.parse_args()
+    global SKIP_CPU
+    SKIP_CPU = args.no_cpu_baseline
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path shards by scan position / detector tile with no exchange; see DESIGN.md)",

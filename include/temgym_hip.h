@@ -48,7 +48,7 @@ int32_t tg_abi_version(void);  /* currently 2 */
  * ======================================================================== */
 
 #define TG_MAX_INSTR 96
-#define TG_MAX_REG 12
+#define TG_MAX_REG 16
 #define TG_MAX_SLOT 10
 #define TG_MAX_DIM 4
 
@@ -119,7 +119,7 @@ typedef struct tg_bin_spec {
   int64_t height, width;
   void* acc;                 /* [height, width] accumulator, row-major      */
   int32_t w_reg;             /* weight register (-1: weight 1)              */
-  int32_t pad;
+  int32_t grid_hint;         /* workgroups to launch (0: built-in heuristic) */
 } tg_bin_spec;
 
 /* Run `prog` over prog->n rays; optionally bin the final positions.
@@ -178,11 +178,22 @@ int tg_stem_forward(const tg_stem_geometry* g, const double* params, const float
 
 /* Descan / geometry fit, forward and adjoint in one pass over the targets.
  *   targets  [n_cam, S, J, 2] f32 target detector coordinates
- *   partials [S, 18 + n_cam + 1] f64 workspace (per-scan-position terms)
+ *   partials [18 + n_cam + 1, S] f64 workspace (per-scan-position terms,
+ *            one row per output entry)
  *   out      [18 + n_cam + 1] f64: d loss / d params, then the loss
  * loss = sum over (c, s, j) of |detector(x, y) - target|^2. */
 int tg_stem_fit(const tg_stem_geometry* g, const double* params, const float* targets, double* partials,
                 double* out, void* stream);
+
+/* ======================================================================== *
+ * Measurement support
+ * ======================================================================== */
+
+/* Streams `bytes` (multiple of 16) from `src` with the loads the streaming
+ * kernels use and writes one partial sum per workgroup to block_sums[n_blocks].
+ * Used by bench.py to report the achievable HBM read rate of the device.
+ * No reference counterpart (measurement only). */
+int tg_hbm_read_probe(const void* src, int64_t bytes, float* block_sums, int32_t n_blocks, void* stream);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libtemgym_hip.so")
 ABI_VERSION = 2
 
 TG_MAX_INSTR = 96
-TG_MAX_REG = 12
+TG_MAX_REG = 16
 TG_MAX_SLOT = 10
 TG_MAX_DIM = 4
 
@@ -73,7 +73,7 @@ class BinSpec(ctypes.Structure):
         ("m", ctypes.c_double * 6),
         ("height", ctypes.c_int64), ("width", ctypes.c_int64),
         ("acc", ctypes.c_void_p),
-        ("w_reg", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("w_reg", ctypes.c_int32), ("grid_hint", ctypes.c_int32),
     ]
 
 
@@ -84,6 +84,8 @@ _SIGNATURES: dict[str, tuple] = {
     "tg_abi_version": (ctypes.c_int32, []),
     "tg_device_cu_count": (ctypes.c_int32, []),
     "tg_ray_program_run": (ctypes.c_int, [ctypes.POINTER(Program), ctypes.POINTER(BinSpec), ctypes.c_void_p]),
+    "tg_hbm_read_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32,
+                                         ctypes.c_void_p]),
 }
 
 _lock = threading.Lock()

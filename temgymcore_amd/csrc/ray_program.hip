@@ -13,9 +13,12 @@
 //   * the program is a kernel argument (kernarg segment -> scalar loads), so
 //     opcode dispatch is a uniform scalar branch and never diverges;
 //   * registers are named VGPRs selected through uniform switches (no scratch);
-//   * binning of small detectors (<= 16384 bins of int counts) goes through
-//     an LDS-privatised histogram flushed once per workgroup; larger detectors
-//     and float accumulators use device-scope atomics.
+//   * binning of small detectors (<= 32768 bins of int counts) goes through
+//     an LDS-privatised histogram of packed 16-bit counters (two bins per
+//     32-bit word; a workgroup bins at most 65535 rays, so a counter never
+//     carries into its neighbour), flushed once per workgroup with
+//     device-scope atomics; larger detectors and float accumulators use
+//     device-scope atomics directly.
 // Arithmetic is exactly one IEEE operation per instruction, built with
 // -ffp-contract=off, so results are bit-identical to the same operation
 // sequence evaluated elementwise on the CPU.
@@ -24,7 +27,8 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int64_t kLdsBins = 16384;
+constexpr int64_t kLdsBins = 32768;
+constexpr int64_t kMaxRaysPerGroup = 65535;
 
 template <typename T>
 __device__ __forceinline__ T imm_of(const tg_instr& in);
@@ -33,25 +37,19 @@ __device__ __forceinline__ float imm_of<float>(const tg_instr& in) { return in.i
 template <>
 __device__ __forceinline__ double imm_of<double>(const tg_instr& in) { return in.imm; }
 
-// The register file is twelve separately declared local arrays, selected by a
-// uniform switch.  One indexed array (or a struct of arrays, which is a single
-// allocation) is merged back by the optimiser into a dynamically indexed
-// scratch access; separate locals stay in VGPRs.
-#define TG_CP(dst, src) _Pragma("unroll") for (int u_ = 0; u_ < R; ++u_) dst[u_] = src[u_];
-#define TG_RD(idx, o)                                                                     \
-  switch (idx) {                                                                          \
-    case 0: TG_CP(o, r0) break;  case 1: TG_CP(o, r1) break;  case 2: TG_CP(o, r2) break;   \
-    case 3: TG_CP(o, r3) break;  case 4: TG_CP(o, r4) break;  case 5: TG_CP(o, r5) break;   \
-    case 6: TG_CP(o, r6) break;  case 7: TG_CP(o, r7) break;  case 8: TG_CP(o, r8) break;   \
-    case 9: TG_CP(o, r9) break;  case 10: TG_CP(o, r10) break; default: TG_CP(o, r11) break; \
-  }
-#define TG_WR(idx, v)                                                                     \
-  switch (idx) {                                                                          \
-    case 0: TG_CP(r0, v) break;  case 1: TG_CP(r1, v) break;  case 2: TG_CP(r2, v) break;   \
-    case 3: TG_CP(r3, v) break;  case 4: TG_CP(r4, v) break;  case 5: TG_CP(r5, v) break;   \
-    case 6: TG_CP(r6, v) break;  case 7: TG_CP(r7, v) break;  case 8: TG_CP(r8, v) break;   \
-    case 9: TG_CP(r9, v) break;  case 10: TG_CP(r10, v) break; default: TG_CP(r11, v) break; \
-  }
+// The register file of each ray is one 16-wide vector value.  Indexing it
+// with a uniform (scalar) register number lowers to the VGPR-indexing mode
+// (s_set_gpr_idx_on / v_movrel*): no scratch memory and no selection trees.
+template <typename T>
+struct RegVec;
+template <>
+struct RegVec<float> {
+  typedef float type __attribute__((ext_vector_type(16)));
+};
+template <>
+struct RegVec<double> {
+  typedef double type __attribute__((ext_vector_type(16)));
+};
 
 template <typename T>
 __device__ __forceinline__ T sign_of(T a) {
@@ -102,22 +100,25 @@ __device__ __forceinline__ void global_deposit(const tg_bin_spec& B, int64_t p, 
 }
 
 // BIN: 0 = no binning, 1 = LDS-privatised integer histogram, 2 = global atomics.
-template <typename T, int R, bool CONTIG, int BIN>
-__global__ __launch_bounds__(kThreads) void ray_program_kernel(const tg_program P, const tg_bin_spec B) {
+// NT: workgroup size (LDS-histogram launches use 1024 so that 16 wavefronts
+// share one private histogram and fewer flushes hit the same global bins).
+template <typename T, int R, bool CONTIG, int BIN, int NT>
+__global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, const tg_bin_spec B) {
+  constexpr int kThreads = NT;
   extern __shared__ int32_t lds_hist[];
   const int64_t n = P.n;
   int64_t hw = 0;
   if constexpr (BIN == 1) {
     hw = B.height * B.width;
-    for (int64_t i = threadIdx.x; i < hw; i += kThreads) lds_hist[i] = 0;
+    for (int64_t i = threadIdx.x; i < (hw + 1) / 2; i += kThreads) lds_hist[i] = 0;
     __syncthreads();
   }
   const int64_t chunk = (int64_t)kThreads * R;
   for (int64_t base = (int64_t)blockIdx.x * chunk; base < n; base += (int64_t)gridDim.x * chunk) {
-    T r0[R], r1[R], r2[R], r3[R], r4[R], r5[R], r6[R], r7[R], r8[R], r9[R], r10[R], r11[R];
+    typedef typename RegVec<T>::type VT;
+    VT r[R];
 #pragma unroll
-    for (int u = 0; u < R; ++u)
-      r0[u] = r1[u] = r2[u] = r3[u] = r4[u] = r5[u] = r6[u] = r7[u] = r8[u] = r9[u] = r10[u] = r11[u] = T(0);
+    for (int u = 0; u < R; ++u) r[u] = (VT)(T(0));
     int64_t idx[R];
     bool ok[R];
 #pragma unroll
@@ -137,13 +138,19 @@ __global__ __launch_bounds__(kThreads) void ray_program_kernel(const tg_program 
         else
           v[u] = ok[u] ? ((const T*)P.slot[s].ptr)[off] : T(0);
       }
-      TG_WR(P.slot[s].reg, v)
+      const int dst = P.slot[s].reg;
+#pragma unroll
+      for (int u = 0; u < R; ++u) r[u][dst] = v[u];
     }
     for (int i = 0; i < P.n_instr; ++i) {
       const tg_instr in = P.instr[i];
+      const int ia = in.a, ib = in.b, id = in.dst;
       T a[R], b[R], d[R];
-      TG_RD(in.a, a)
-      TG_RD(in.b, b)
+#pragma unroll
+      for (int u = 0; u < R; ++u) {
+        a[u] = r[u][ia];
+        b[u] = r[u][ib];
+      }
       const bool f32op = sizeof(T) == 8 && (in.op & TG_OP_F32);
       const T im = f32op ? (T)in.immf : imm_of<T>(in);
       switch (in.op & 0x7f) {
@@ -208,11 +215,14 @@ __global__ __launch_bounds__(kThreads) void ray_program_kernel(const tg_program 
 #pragma unroll
         for (int u = 0; u < R; ++u) d[u] = (T)(float)d[u];
       }
-      TG_WR(in.dst, d)
+#pragma unroll
+      for (int u = 0; u < R; ++u) r[u][id] = d[u];
     }
     for (int k = 0; k < P.n_out; ++k) {
       T v[R];
-      TG_RD(P.out_reg[k], v)
+      const int src = P.out_reg[k];
+#pragma unroll
+      for (int u = 0; u < R; ++u) v[u] = r[u][src];
       if (sizeof(T) == 8 && P.out_dtype[k] == TG_F32) {
         float* __restrict__ o = (float*)P.out[k];
 #pragma unroll
@@ -228,19 +238,25 @@ __global__ __launch_bounds__(kThreads) void ray_program_kernel(const tg_program 
     if constexpr (BIN != 0) {
       T xv[R], yv[R], wv[R];
       if (B.x_reg >= 0) {
-        TG_RD(B.x_reg, xv)
+        const int src = B.x_reg;
+#pragma unroll
+        for (int u = 0; u < R; ++u) xv[u] = r[u][src];
       } else {
 #pragma unroll
         for (int u = 0; u < R; ++u) xv[u] = (T)B.x_uniform;
       }
       if (B.y_reg >= 0) {
-        TG_RD(B.y_reg, yv)
+        const int src = B.y_reg;
+#pragma unroll
+        for (int u = 0; u < R; ++u) yv[u] = r[u][src];
       } else {
 #pragma unroll
         for (int u = 0; u < R; ++u) yv[u] = (T)B.y_uniform;
       }
       if (B.w_reg >= 0) {
-        TG_RD(B.w_reg, wv)
+        const int src = B.w_reg;
+#pragma unroll
+        for (int u = 0; u < R; ++u) wv[u] = r[u][src];
       } else {
 #pragma unroll
         for (int u = 0; u < R; ++u) wv[u] = T(1);
@@ -251,7 +267,7 @@ __global__ __launch_bounds__(kThreads) void ray_program_kernel(const tg_program 
         const int64_t p = bin_index(B, (double)yv[u], (double)xv[u]);
         if (p < 0) continue;
         if constexpr (BIN == 1) {
-          atomicAdd(&lds_hist[p], 1);
+          atomicAdd(&lds_hist[p >> 1], 1 << (16 * (int)(p & 1)));
         } else {
           global_deposit<T>(B, p, wv[u]);
         }
@@ -260,28 +276,44 @@ __global__ __launch_bounds__(kThreads) void ray_program_kernel(const tg_program 
   }
   if constexpr (BIN == 1) {
     __syncthreads();
-    for (int64_t i = threadIdx.x; i < hw; i += kThreads) {
-      const int32_t c = lds_hist[i];
-      if (c == 0) continue;
-      if (B.acc_dtype == 0)
-        atomicAdd((int32_t*)B.acc + i, c);
-      else
-        atomicAdd((unsigned long long*)B.acc + i, (unsigned long long)(int64_t)c);
+    for (int64_t i = threadIdx.x; i < (hw + 1) / 2; i += kThreads) {
+      const uint32_t w = (uint32_t)lds_hist[i];
+      if (w == 0) continue;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int32_t c = (int32_t)((w >> (16 * h)) & 0xffffu);
+        const int64_t bin = 2 * i + h;
+        if (c == 0 || bin >= hw) continue;
+        if (B.acc_dtype == 0)
+          atomicAdd((int32_t*)B.acc + bin, c);
+        else
+          atomicAdd((unsigned long long*)B.acc + bin, (unsigned long long)(int64_t)c);
+      }
     }
   }
 }
 
 template <typename T, int R, bool CONTIG, int BIN>
 int launch(const tg_program& P, const tg_bin_spec& B, hipStream_t stream) {
+  constexpr int kThreads = (BIN == 1) ? 1024 : 256;
   const int64_t chunk = (int64_t)kThreads * R;
   const int64_t nchunks = tg::cdiv(P.n, chunk);
   const int cus = tg::cu_count();
-  // LDS histograms are zeroed and flushed once per workgroup, so keep the
-  // grid near two workgroups per CU; streaming variants get more waves.
-  const int64_t cap = (BIN == 1) ? 2LL * cus : 16LL * cus;
-  const int grid = (int)(nchunks < cap ? nchunks : cap);
-  const size_t lds = (BIN == 1) ? (size_t)(B.height * B.width) * sizeof(int32_t) : 0;
-  hipLaunchKernelGGL((ray_program_kernel<T, R, CONTIG, BIN>), dim3(grid), dim3(kThreads), lds, stream, P, B);
+  // LDS histograms are zeroed and flushed once per workgroup (and every flush
+  // adds into the same global bins), so that grid stays at one 1024-thread
+  // workgroup per CU; streaming variants get more waves.  A caller (the
+  // autotuner) may override.
+  int64_t cap = (BIN == 1) ? 1LL * cus : 16LL * cus;
+  if (B.grid_hint > 0) cap = B.grid_hint;
+  int64_t grid = nchunks < cap ? nchunks : cap;
+  if (BIN == 1) {
+    // packed 16-bit counters: at most kMaxRaysPerGroup rays per workgroup
+    const int64_t min_grid = tg::cdiv(P.n, (kMaxRaysPerGroup / chunk) * chunk);
+    if (grid < min_grid) grid = min_grid;
+  }
+  const size_t lds = (BIN == 1) ? (size_t)((B.height * B.width + 1) / 2) * sizeof(int32_t) : 0;
+  hipLaunchKernelGGL((ray_program_kernel<T, R, CONTIG, BIN, kThreads>), dim3((unsigned)grid), dim3(kThreads), lds,
+                     stream, P, B);
   TG_LAUNCH_CHECK("ray_program_kernel launch");
   return TG_OK;
 }
@@ -348,6 +380,7 @@ extern "C" int tg_ray_program_run(const tg_program* prog, const tg_bin_spec* bin
                    B.w_reg >= -1 && B.w_reg < TG_MAX_REG,
                "tg_ray_program_run: bin register out of range");
     TG_REQUIRE(B.acc_dtype >= 2 || B.w_reg < 0, "tg_ray_program_run: weights need a float accumulator");
+    TG_REQUIRE(B.grid_hint >= 0, "tg_ray_program_run: negative grid_hint");
     const bool small = B.height * B.width <= kLdsBins;
     mode = (B.acc_dtype <= 1 && small) ? 1 : 2;
   }

@@ -161,77 +161,31 @@ __device__ __forceinline__ float wave_sum(float v) {
 }
 
 constexpr int kMom = 7;  // sum rx, rx*tx, rx*ty, ry, ry*tx, ry*ty, |r|^2
+constexpr int kPerLane = 8;
+constexpr int64_t kTile = (int64_t)kThreads * kPerLane;
 
-__global__ __launch_bounds__(kThreads) void stem_fit_kernel(const tg_stem_geometry G, const double* __restrict__ P,
-                                                            const float* __restrict__ T,
-                                                            double* __restrict__ partials) {
-  __shared__ float th[kMaxSide];
-  __shared__ double mom[TG_STEM_MAX_CAM][kWaves][kMom];
-  const int64_t s = blockIdx.x;
-  const int64_t S = (int64_t)G.scan_h * G.scan_w;
-  const int64_t J = (int64_t)G.n_side * G.n_side;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k = threadIdx.x; k < G.n_side; k += kThreads) th[k] = slope_of(G, k);
-  const ScanPos p = scan_position(G, P, s);
-  __syncthreads();
-  const int64_t npair = J >> 1;  // two rays per float4 of targets (n_side even)
-  const int half = G.n_side >> 1, drow = kThreads / half, dcol = kThreads % half;
-  for (int c = 0; c < G.n_cam; ++c) {
-    const double L = P[18 + c];
-    double bx, by;
-    offsets_at(G, P, p, L, bx, by);
-    // double-float split of the slope scale and offsets: the residual
-    //   r = th*a + b - t = (fma(th, a_hi, -t) + b_hi) + fma(th, a_lo, b_lo)
-    // is exact to about one ulp of r itself (fma(th, a_hi, -t) rounds once),
-    // not to one ulp of the ~1e-4 m detector coordinate.
-    const double ad = G.defocus + L;
-    const float a_hi = (float)ad, a_lo = (float)(ad - (double)a_hi);
-    const float bx_hi = (float)bx, bx_lo = (float)(bx - (double)bx_hi);
-    const float by_hi = (float)by, by_lo = (float)(by - (double)by_hi);
-    const float4* __restrict__ t4 = (const float4*)(T + ((int64_t)c * S + s) * J * 2);
-    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f, m6 = 0.f;
-    // ray pair q = (row iy, columns 2*hc, 2*hc+1); stepped without division
-    int iy = (int)threadIdx.x / half, hc = (int)threadIdx.x % half;
-#pragma unroll 8
-    for (int64_t q = threadIdx.x; q < npair; q += kThreads) {
-      const float4 t = t4[q];
-      const float tx0 = th[2 * hc], tx1 = th[2 * hc + 1], ty = th[iy];
-      hc += dcol;
-      iy += drow;
-      if (hc >= half) {
-        hc -= half;
-        ++iy;
-      }
-      const float rx0 = (fmaf(tx0, a_hi, -t.x) + bx_hi) + fmaf(tx0, a_lo, bx_lo);
-      const float ry0 = (fmaf(ty, a_hi, -t.y) + by_hi) + fmaf(ty, a_lo, by_lo);
-      const float rx1 = (fmaf(tx1, a_hi, -t.z) + bx_hi) + fmaf(tx1, a_lo, bx_lo);
-      const float ry1 = (fmaf(ty, a_hi, -t.w) + by_hi) + fmaf(ty, a_lo, by_lo);
-      m0 += rx0 + rx1;
-      m1 = fmaf(rx0, tx0, fmaf(rx1, tx1, m1));
-      m2 = fmaf(rx0 + rx1, ty, m2);
-      m3 += ry0 + ry1;
-      m4 = fmaf(ry0, tx0, fmaf(ry1, tx1, m4));
-      m5 = fmaf(ry0 + ry1, ty, m5);
-      m6 = fmaf(rx0, rx0, fmaf(ry0, ry0, fmaf(rx1, rx1, fmaf(ry1, ry1, m6))));
-    }
-    m0 = wave_sum(m0);
-    m1 = wave_sum(m1);
-    m2 = wave_sum(m2);
-    m3 = wave_sum(m3);
-    m4 = wave_sum(m4);
-    m5 = wave_sum(m5);
-    m6 = wave_sum(m6);
-    if (lane == 0) {
-      mom[c][wave][0] = m0;
-      mom[c][wave][1] = m1;
-      mom[c][wave][2] = m2;
-      mom[c][wave][3] = m3;
-      mom[c][wave][4] = m4;
-      mom[c][wave][5] = m5;
-      mom[c][wave][6] = m6;
-    }
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Tile `it` (camera c = it / ntile, tile it % ntile) of scan position s:
+// kPerLane float4 per lane, lane-strided so each load instruction of a
+// wavefront covers 1 KiB of contiguous targets.  Streamed once: non-temporal.
+__device__ __forceinline__ void load_tile(const f4* __restrict__ T4, int64_t S, int64_t s, int64_t npair,
+                                          int64_t ntile, int64_t it, f4 (&dst)[kPerLane]) {
+  const int64_t c = it / ntile, tile = it - c * ntile;
+  const int64_t row = (c * S + s) * npair;
+  const int64_t q0 = tile * kTile + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < kPerLane; ++k) {
+    const int64_t q = q0 + (int64_t)k * kThreads;
+    dst[k] = q < npair ? __builtin_nontemporal_load(T4 + row + q) : f4{0.f, 0.f, 0.f, 0.f};
   }
-  __syncthreads();
+}
+
+// Per-scan-position gradient terms from the reduced moments M[c][k]
+// (thread t < np writes row t of the workspace column s).
+__device__ __forceinline__ void fit_epilogue(const tg_stem_geometry& G, const double* __restrict__ P,
+                                             const ScanPos& p, const double (&mom)[TG_STEM_MAX_CAM][kWaves][kMom],
+                                             int64_t s, int64_t S, double* __restrict__ partials) {
   const int np = 18 + G.n_cam + 1;
   const int t = threadIdx.x;
   if (t >= np) return;
@@ -244,6 +198,7 @@ __global__ __launch_bounds__(kThreads) void stem_fit_kernel(const tg_stem_geomet
 #pragma unroll
     for (int k = 0; k < kMom; ++k) {
       double acc = 0.0;
+#pragma unroll
       for (int w = 0; w < kWaves; ++w) acc += mom[c][w][k];
       M[k] = acc;
     }
@@ -276,23 +231,130 @@ __global__ __launch_bounds__(kThreads) void stem_fit_kernel(const tg_stem_geomet
       g += M[6];
     }
   }
-  partials[s * np + t] = g;
+  partials[(int64_t)t * S + s] = g;
 }
 
-// Fixed-order FP64 column sums of the [rows, np] workspace.
-__global__ __launch_bounds__(kThreads) void column_sum_kernel(const double* __restrict__ partials, int64_t rows,
-                                                              int np, double* __restrict__ out) {
-  __shared__ double red[kThreads];
-  const int col = blockIdx.x;
-  double acc = 0.0;
-  for (int64_t r = threadIdx.x; r < rows; r += kThreads) acc += partials[r * np + col];
-  red[threadIdx.x] = acc;
+// One workgroup per scan position.  The targets are streamed in tiles of
+// kPerLane float4 (ray pairs) per lane, double-buffered across tiles and
+// camera lengths, so the per-camera wavefront reductions never leave the load
+// pipeline empty.  (A persistent variant that also prefetched across scan
+// positions needed 140 VGPRs, fell to 3 waves per SIMD and was slower.)
+__global__ __launch_bounds__(kThreads) void stem_fit_kernel(const tg_stem_geometry G, const double* __restrict__ P,
+                                                            const float* __restrict__ T,
+                                                            double* __restrict__ partials) {
+  __shared__ float th[kMaxSide];
+  __shared__ double mom[TG_STEM_MAX_CAM][kWaves][kMom];
+  const int64_t S = (int64_t)G.scan_h * G.scan_w;
+  const int64_t J = (int64_t)G.n_side * G.n_side;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = threadIdx.x; k < G.n_side; k += kThreads) th[k] = slope_of(G, k);
   __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+  const int64_t npair = J >> 1;  // two rays per float4 of targets (n_side even)
+  const int half = G.n_side >> 1, drow = kThreads / half, dcol = kThreads % half;
+  const int64_t ntile = (npair + kTile - 1) / kTile;
+  const int64_t nit = (int64_t)G.n_cam * ntile;
+  const f4* __restrict__ T4 = (const f4*)T;
+  f4 cur[kPerLane], nxt[kPerLane];
+  const int64_t s = blockIdx.x;
+  load_tile(T4, S, s, npair, ntile, 0, cur);
+  {
+    const ScanPos p = scan_position(G, P, s);
+    float a_hi = 0.f, a_lo = 0.f, bx_hi = 0.f, bx_lo = 0.f, by_hi = 0.f, by_lo = 0.f;
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f, m6 = 0.f;
+    for (int64_t it = 0; it < nit; ++it) {
+      if (it + 1 < nit) load_tile(T4, S, s, npair, ntile, it + 1, nxt);
+      const int c = (int)(it / ntile);
+      const int64_t tile = it - (int64_t)c * ntile;
+      if (tile == 0) {
+        const double L = P[18 + c];
+        double bx, by;
+        offsets_at(G, P, p, L, bx, by);
+        // double-float split of the slope scale and offsets: the residual
+        //   r = th*a + b - t = (fma(th, a_hi, -t) + b_hi) + fma(th, a_lo, b_lo)
+        // is exact to about one ulp of r itself (fma(th, a_hi, -t) rounds once),
+        // not to one ulp of the ~1e-4 m detector coordinate.
+        const double ad = G.defocus + L;
+        a_hi = (float)ad;
+        a_lo = (float)(ad - (double)a_hi);
+        bx_hi = (float)bx;
+        bx_lo = (float)(bx - (double)bx_hi);
+        by_hi = (float)by;
+        by_lo = (float)(by - (double)by_hi);
+        m0 = m1 = m2 = m3 = m4 = m5 = m6 = 0.f;
+      }
+      // ray pair q = (row iy, columns 2*hc, 2*hc+1); stepped without division
+      const int64_t q0 = tile * kTile + threadIdx.x;
+      int iy = (int)(q0 / half), hc = (int)(q0 % half);
+#pragma unroll
+      for (int k = 0; k < kPerLane; ++k) {
+        const bool valid = q0 + (int64_t)k * kThreads < npair;
+        if (valid) {
+          const f4 t = cur[k];
+          const float tx0 = th[2 * hc], tx1 = th[2 * hc + 1], ty = th[iy];
+          const float rx0 = (fmaf(tx0, a_hi, -t.x) + bx_hi) + fmaf(tx0, a_lo, bx_lo);
+          const float ry0 = (fmaf(ty, a_hi, -t.y) + by_hi) + fmaf(ty, a_lo, by_lo);
+          const float rx1 = (fmaf(tx1, a_hi, -t.z) + bx_hi) + fmaf(tx1, a_lo, bx_lo);
+          const float ry1 = (fmaf(ty, a_hi, -t.w) + by_hi) + fmaf(ty, a_lo, by_lo);
+          m0 += rx0 + rx1;
+          m1 = fmaf(rx0, tx0, fmaf(rx1, tx1, m1));
+          m2 = fmaf(rx0 + rx1, ty, m2);
+          m3 += ry0 + ry1;
+          m4 = fmaf(ry0, tx0, fmaf(ry1, tx1, m4));
+          m5 = fmaf(ry0 + ry1, ty, m5);
+          m6 = fmaf(rx0, rx0, fmaf(ry0, ry0, fmaf(rx1, rx1, fmaf(ry1, ry1, m6))));
+        }
+        hc += dcol;
+        iy += drow;
+        if (hc >= half) {
+          hc -= half;
+          ++iy;
+        }
+      }
+      if (tile == ntile - 1) {
+        const float w0 = wave_sum(m0), w1 = wave_sum(m1), w2 = wave_sum(m2), w3 = wave_sum(m3);
+        const float w4 = wave_sum(m4), w5 = wave_sum(m5), w6 = wave_sum(m6);
+        if (lane == 0) {
+          mom[c][wave][0] = w0;
+          mom[c][wave][1] = w1;
+          mom[c][wave][2] = w2;
+          mom[c][wave][3] = w3;
+          mom[c][wave][4] = w4;
+          mom[c][wave][5] = w5;
+          mom[c][wave][6] = w6;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kPerLane; ++k) cur[k] = nxt[k];
+    }
     __syncthreads();
+    fit_epilogue(G, P, p, mom, s, S, partials);
   }
-  if (threadIdx.x == 0) out[col] = red[0];
+}
+
+// Fixed-order FP64 row sums of the [np, rows] workspace (one block per row,
+// coalesced reads, wavefront shuffles then one LDS step).
+constexpr int kSumThreads = 1024;
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(kSumThreads) void row_sum_kernel(const double* __restrict__ partials, int64_t rows,
+                                                              double* __restrict__ out) {
+  __shared__ double red[kSumThreads / 64];
+  const double* __restrict__ src = partials + (int64_t)blockIdx.x * rows;
+  double acc = 0.0;
+  for (int64_t r = threadIdx.x; r < rows; r += kSumThreads) acc += src[r];
+  acc = wave_sum_f64(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    double v = threadIdx.x < kSumThreads / 64 ? red[threadIdx.x] : 0.0;
+    v = wave_sum_f64(v);
+    if (threadIdx.x == 0) out[blockIdx.x] = v;
+  }
 }
 
 int validate(const tg_stem_geometry* g, const double* params) {
@@ -337,8 +399,7 @@ extern "C" int tg_stem_fit(const tg_stem_geometry* g, const double* params, cons
   hipLaunchKernelGGL(stem_fit_kernel, dim3((unsigned)S), dim3(kThreads), 0, (hipStream_t)stream, *g, params, targets,
                      partials);
   TG_LAUNCH_CHECK("stem_fit_kernel launch");
-  hipLaunchKernelGGL(column_sum_kernel, dim3((unsigned)np), dim3(kThreads), 0, (hipStream_t)stream, partials, S, np,
-                     out);
-  TG_LAUNCH_CHECK("column_sum_kernel launch");
+  hipLaunchKernelGGL(row_sum_kernel, dim3((unsigned)np), dim3(kSumThreads), 0, (hipStream_t)stream, partials, S, out);
+  TG_LAUNCH_CHECK("row_sum_kernel launch");
   return TG_OK;
 }

@@ -255,7 +255,8 @@ class STEMFitWorkspace:
         self.model = model
         self.geometry = model.geometry()
         self.np = 18 + model.n_cam + 1
-        self.partials = torch.empty((model.n_scan, self.np), dtype=torch.float64, device=device)
+        # [np, S]: per-scan-position contributions, one row per output entry
+        self.partials = torch.empty((self.np, model.n_scan), dtype=torch.float64, device=device)
         self.out = torch.empty(self.np, dtype=torch.float64, device=device)
 
     def run(self, params: torch.Tensor, targets: torch.Tensor) -> torch.Tensor:

@@ -130,7 +130,7 @@ def test_fit_full_size_properties(device):
     assert abs(out0[-1]) < 1e-16 * model.n_scan * model.n_rays * model.n_cam
     targets = _targets(model, device)
     out = ws.run(params, targets).cpu().numpy()
-    rows = ws.partials.cpu().numpy()
+    rows = ws.partials.cpu().numpy().T  # [S, np]
     np.testing.assert_allclose(rows.sum(axis=0), out, rtol=1e-12, atol=0)
     pick = np.random.default_rng(0).choice(model.n_scan, 6, replace=False)
     t = targets.cpu().numpy()[:, pick]
@@ -167,4 +167,4 @@ def test_autograd_function_and_graph_replay(device):
         graph.replay()
         torch.cuda.synchronize()
         torch.testing.assert_close(ws.out, eager, rtol=0, atol=0)
-        _ = (targets[0, 0, 0] * 1.0).item()  # eager work between replays
+        _ = (targets[0, 0, 0, 0] * 1.0).item()  # eager work between replays
