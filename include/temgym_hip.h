@@ -186,6 +186,55 @@ int tg_stem_fit(const tg_stem_geometry* g, const double* params, const float* ta
                 double* out, void* stream);
 
 /* ======================================================================== *
+ * Gaussian beamlets (reference gaussian.py:225-427)
+ *
+ * Every beamlet contributes exp(Phi(px, py)) to detector pixel (py, px),
+ * where Phi is a complex quadratic in the pixel indices obtained by folding
+ * the reference's per-beamlet 2x2 complex algebra (_beam_field,
+ * gaussian.py:276-316, or _input_beam_field, gaussian.py:402-407), the
+ * amplitude prefactor, the k*pathlength phase and the pixel->metre transform
+ * of the Grid into six complex coefficients (computed in f64).
+ * ======================================================================== */
+
+/* Per-beamlet parameter row (f64, TG_BEAMLET_NPARAM columns):
+ * x, y, dx, dy, pathlength, amp_re, amp_im, waist_x, waist_y, radius_x,
+ * radius_y (inf allowed), wavelength, theta, 0, 0, 0. */
+#define TG_BEAMLET_NPARAM 16
+
+enum tg_beamlet_mode {
+  TG_BEAMLET_PROPAGATED = 0,  /* make_gaussian_image: field after a 5x5 ABCD   */
+  TG_BEAMLET_INPUT = 1        /* evaluate_gaussian_input_image: input plane    */
+};
+
+typedef struct tg_beamlet_coef {
+  double phi[12];      /* (re, im) of a, b, c, d, e, f:
+                          Phi = a px^2 + b px py + c py^2 + d px + e py + f      */
+  float k_re, k_im;    /* exp(2a): ratio of successive x-step ratios             */
+  float cx, cy;        /* envelope centre (pixels)                               */
+  float s00, s01, s11; /* envelope form S: |field| = peak * exp(-(p-c)^T S (p-c)) */
+  float cut;           /* contributions kept where (p-c)^T S (p-c) <= cut        */
+  float box_w, box_h;  /* half-extents of that ellipse (pixels; huge: unbounded) */
+} tg_beamlet_coef;     /* 144 bytes */
+
+/* Coefficients of n beamlets.
+ *   params  [n, TG_BEAMLET_NPARAM] f64 (device)
+ *   abcd    [n or 1, 5, 5] f64 ray-transfer matrices (propagated mode; NULL
+ *           in input mode); abcd_stride = 25 (per beamlet) or 0 (shared)
+ *   pix2m   [6] f64: y_m = m0*py + m1*px + m2, x_m = m3*py + m4*px + m5
+ *   nsigma  cutoff: contributions below exp(-nsigma^2) of the beamlet's peak
+ *           amplitude are dropped (4 = the "4 sigma" cut; <= 0: keep down to
+ *           exp(-80), i.e. below f32 resolution). */
+int tg_beamlet_coefficients(const double* params, int64_t n, const double* abcd, int64_t abcd_stride, int32_t mode,
+                            const double* pix2m, double nsigma, tg_beamlet_coef* out, void* stream);
+
+/* field[H, W] (complex64 as interleaved f32 pairs) = sum over beamlets of
+ * exp(Phi_b(px, py)), overwritten.  `workspace` needs
+ * tg_beamlet_sum_workspace(n, H, W) bytes. */
+int64_t tg_beamlet_sum_workspace(int64_t n, int64_t height, int64_t width);
+int tg_beamlet_sum(const tg_beamlet_coef* coef, int64_t n, int64_t height, int64_t width, float* field,
+                   void* workspace, int64_t workspace_bytes, void* stream);
+
+/* ======================================================================== *
  * Measurement support
  * ======================================================================== */
 

@@ -1,0 +1,253 @@
+"""Gaussian beamlets (reference gaussian.py:13-427).
+
+``make_gaussian_image`` and ``evaluate_gaussian_input_image`` keep the
+reference signatures; ``batch_size`` is accepted and ignored (the kernels
+tile the detector themselves).  Both return a complex64 device tensor of the
+grid's shape.  ``cutoff`` (in units of the beamlet's envelope: contributions
+below exp(-cutoff^2) of a beamlet's peak amplitude are skipped) defaults to
+None, which keeps everything down to exp(-80) -- below f32 resolution, i.e.
+the reference's uncut sum.  ``cutoff=4.0`` is the "4 sigma" cut.
+
+The per-beamlet 5x5 ray-transfer matrix comes from ``transfer.jacobian`` of
+``run_to_end`` over the model (reference gaussian.py:234-238 takes
+``jax.jacobian``), evaluated once per distinct input-plane z -- exact for
+models built from the affine elements (every element except Biprism).
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+import torch
+
+from . import _lib
+from .ray import Ray
+
+NPARAM = 16
+
+
+class BeamletCoef(ctypes.Structure):
+    _fields_ = [
+        ("phi", ctypes.c_double * 12),
+        ("k_re", ctypes.c_float), ("k_im", ctypes.c_float),
+        ("cx", ctypes.c_float), ("cy", ctypes.c_float),
+        ("s00", ctypes.c_float), ("s01", ctypes.c_float), ("s11", ctypes.c_float),
+        ("cut", ctypes.c_float),
+        ("box_w", ctypes.c_float), ("box_h", ctypes.c_float),
+    ]
+
+
+_P = ctypes.c_void_p
+_lib.register_signatures({
+    "tg_beamlet_coefficients": (ctypes.c_int, [_P, ctypes.c_int64, _P, ctypes.c_int64, ctypes.c_int32, _P,
+                                               ctypes.c_double, _P, _P]),
+    "tg_beamlet_sum_workspace": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
+    "tg_beamlet_sum": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _P, _P, ctypes.c_int64,
+                                      _P]),
+})
+
+MODE_PROPAGATED = 0
+MODE_INPUT = 1
+
+
+# ---- closed forms (reference gaussian.py:13-32, :92-110) -----------------------
+
+def w_z(w0, z, z_r):
+    return w0 * np.sqrt(1 + (z / z_r) ** 2)
+
+
+def zR(w0, wavelength):
+    return (np.pi * w0**2) / wavelength
+
+
+def R(z, z_r):
+    if abs(z) < 1e-10:
+        return np.inf
+    return z * (1 + (z_r / z) ** 2)
+
+
+def gaussian_beam(x, y, q_inv, k, offset_x=0, offset_y=0):
+    return np.exp(1j * k * ((x + offset_x) ** 2 + (y + offset_y) ** 2) / 2 * q_inv)
+
+
+def Qinv_ABCD(Qinv, A, B, C, D):
+    """solve(A + B Qinv, C + D Qinv) (reference gaussian.py:92-96)."""
+    return np.linalg.solve(A + B @ Qinv, C + D @ Qinv)
+
+
+def q_inv(z, w0, wl):
+    z_r = zR(w0, wl)
+    if abs(z) < 1e-10:
+        return 1j * wl / (np.pi * w0**2)
+    return -1.0 / R(z, z_r) + 1j * wl / (np.pi * w_z(w0, z, z_r) ** 2)
+
+
+# ---- GaussianRay -------------------------------------------------------------------
+
+@dataclasses.dataclass(frozen=True, kw_only=True)
+class GaussianRay(Ray):
+    """A Ray carrying a Gaussian beamlet (reference gaussian.py:113-177)."""
+
+    amplitude: object
+    waist_xy: object
+    radii_of_curv: object
+    wavelength: object
+    theta: object
+
+    def derive(self, **updates):
+        """Copy with fields replaced; values may be callables of self."""
+        vals = {f.name: getattr(self, f.name) for f in dataclasses.fields(self)}
+        for k, v in updates.items():
+            vals[k] = v(self) if callable(v) else v
+        return GaussianRay(**vals)
+
+    def to_ray(self) -> Ray:
+        return Ray(x=self.x, y=self.y, dx=self.dx, dy=self.dy, z=self.z, pathlength=self.pathlength, _one=self._one)
+
+    @property
+    def q_inv(self):
+        """(1/q_x, 1/q_y) per principal axis (numpy complex128)."""
+        w = np.asarray(_host(self.waist_xy), dtype=np.float64).reshape(-1, 2)
+        r = np.asarray(_host(self.radii_of_curv), dtype=np.float64).reshape(-1, 2)
+        wl = np.asarray(_host(self.wavelength), dtype=np.float64)
+        im = 1j * wl[..., None] / (np.pi * w**2) if wl.ndim else 1j * wl / (np.pi * w**2)
+        with np.errstate(divide="ignore"):
+            q = np.where(np.isinf(r), im, -1.0 / r + im)
+        return q[:, 0], q[:, 1]
+
+    @property
+    def Q_inv(self):
+        """R(theta) diag(1/qx, 1/qy) R(theta)^T, shape (n, 2, 2)."""
+        qx, qy = self.q_inv
+        th = np.broadcast_to(np.asarray(_host(self.theta), dtype=np.float64), qx.shape)
+        c, s = np.cos(th), np.sin(th)
+        Rm = np.stack([np.stack([c, -s], -1), np.stack([s, c], -1)], -2)
+        D = np.zeros(qx.shape + (2, 2), dtype=np.complex128)
+        D[..., 0, 0] = qx
+        D[..., 1, 1] = qy
+        return Rm @ D @ np.swapaxes(Rm, -1, -2)
+
+
+def _host(v):
+    if isinstance(v, torch.Tensor):
+        return v.detach().cpu().numpy()
+    return v
+
+
+# ---- device plumbing -----------------------------------------------------------------
+
+def _col(v, n, device, dtype=torch.float64):
+    t = v if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v))
+    t = t.to(device=device)
+    if t.is_complex():
+        raise TypeError("complex value in a real beamlet parameter")
+    return t.to(dtype).reshape(-1).expand(n) if t.numel() == 1 else t.to(dtype).reshape(n)
+
+
+def pack_params(rays: GaussianRay, device) -> torch.Tensor:
+    """[n, 16] float64 parameter rows of the beamlets (see include/temgym_hip.h)."""
+    x = rays.x if isinstance(rays.x, torch.Tensor) else torch.as_tensor(np.asarray(rays.x))
+    n = int(x.numel())
+    amp = rays.amplitude
+    amp = amp if isinstance(amp, torch.Tensor) else torch.as_tensor(np.asarray(amp))
+    amp = amp.to(device).reshape(-1)
+    if amp.numel() == 1:
+        amp = amp.expand(n)
+    amp_re = amp.real.to(torch.float64) if amp.is_complex() else amp.to(torch.float64)
+    amp_im = amp.imag.to(torch.float64) if amp.is_complex() else torch.zeros_like(amp_re)
+    w = rays.waist_xy if isinstance(rays.waist_xy, torch.Tensor) else torch.as_tensor(np.asarray(rays.waist_xy))
+    r = rays.radii_of_curv if isinstance(rays.radii_of_curv, torch.Tensor) else torch.as_tensor(
+        np.asarray(rays.radii_of_curv))
+    w = w.to(device=device, dtype=torch.float64).reshape(-1, 2).expand(n, 2)
+    r = r.to(device=device, dtype=torch.float64).reshape(-1, 2).expand(n, 2)
+    cols = [
+        _col(rays.x, n, device), _col(rays.y, n, device), _col(rays.dx, n, device), _col(rays.dy, n, device),
+        _col(rays.pathlength, n, device), amp_re, amp_im, w[:, 0], w[:, 1], r[:, 0], r[:, 1],
+        _col(rays.wavelength, n, device), _col(rays.theta, n, device),
+    ]
+    zeros = torch.zeros(n, dtype=torch.float64, device=device)
+    cols += [zeros] * (NPARAM - len(cols))
+    return torch.stack(cols, dim=1).contiguous()
+
+
+def _pix2m(grid) -> np.ndarray:
+    T = grid.pixels_to_metres_mat
+    return np.array([T[0, 0], T[0, 1], T[0, 2], T[1, 0], T[1, 1], T[1, 2]], dtype=np.float64)
+
+
+def _abcd_per_beamlet(rays: GaussianRay, model, device, n: int):
+    """(abcd tensor [m, 5, 5] f64, stride) of the model for each beamlet's z."""
+    from .components import Biprism
+    from .transfer import ray_matrix
+
+    if any(isinstance(c, Biprism) for c in model):
+        raise NotImplementedError("make_gaussian_image: per-beamlet ABCD of nonlinear elements (Biprism)")
+    z = rays.z if isinstance(rays.z, torch.Tensor) else torch.as_tensor(np.asarray(rays.z, dtype=np.float64))
+    z = z.reshape(-1).to(torch.float64)
+    if n == 0:
+        return torch.zeros((1, 5, 5), dtype=torch.float64, device=device), 0
+    uniq, inv = torch.unique(z.cpu(), return_inverse=True)
+    mats = np.stack([ray_matrix(model, Ray(0.0, 0.0, 0.0, 0.0, float(zv), 0.0)) for zv in uniq.tolist()])
+    if len(mats) == 1:
+        return torch.from_numpy(mats).to(device), 0
+    idx = inv.reshape(-1).expand(n) if inv.numel() == 1 else inv.reshape(-1)
+    return torch.from_numpy(mats[idx.numpy()]).to(device).contiguous(), 25
+
+
+def beamlet_coefficients(params: torch.Tensor, grid, mode: int, abcd: torch.Tensor | None = None,
+                         abcd_stride: int = 0, cutoff: float | None = None) -> torch.Tensor:
+    """Run the f64 coefficient kernel; returns a uint8 tensor of n packed records."""
+    lib = _lib.load()
+    n = int(params.shape[0])
+    out = torch.empty(n * ctypes.sizeof(BeamletCoef), dtype=torch.uint8, device=params.device)
+    m = _pix2m(grid)
+    st = lib.tg_beamlet_coefficients(params.data_ptr(), n, abcd.data_ptr() if abcd is not None else None,
+                                     abcd_stride, mode, m.ctypes.data, float(cutoff or 0.0), out.data_ptr(),
+                                     _lib.stream_ptr(params.device))
+    _lib.check(st, "tg_beamlet_coefficients")
+    return out
+
+
+class BeamletSum:
+    """Reusable buffers of one (n, detector shape) summation."""
+
+    def __init__(self, n: int, shape, device):
+        lib = _lib.load()
+        self.n = int(n)
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.ws_bytes = int(lib.tg_beamlet_sum_workspace(self.n, *self.shape))
+        self.workspace = torch.empty(max(self.ws_bytes, 16), dtype=torch.uint8, device=device)
+        self.field = torch.empty(self.shape + (2,), dtype=torch.float32, device=device)
+
+    def run(self, coef: torch.Tensor) -> torch.Tensor:
+        st = _lib.load().tg_beamlet_sum(coef.data_ptr(), self.n, self.shape[0], self.shape[1],
+                                         self.field.data_ptr(), self.workspace.data_ptr(), self.ws_bytes,
+                                         _lib.stream_ptr(self.field.device))
+        _lib.check(st, "tg_beamlet_sum")
+        return torch.view_as_complex(self.field)
+
+
+def make_gaussian_image(gaussian_rays: GaussianRay, model, batch_size=128, cutoff: float | None = None):
+    """Complex field of all beamlets after `model` on its last element (a Grid)."""
+    from .grid import Grid
+
+    rays = gaussian_rays
+    assert isinstance(rays, GaussianRay)
+    grid = model[-1]
+    assert isinstance(grid, Grid)
+    device = _lib.require_device()
+    params = pack_params(rays.to_vector(), device)
+    abcd, stride = _abcd_per_beamlet(rays, model, device, int(params.shape[0]))
+    coef = beamlet_coefficients(params, grid, MODE_PROPAGATED, abcd, stride, cutoff)
+    return BeamletSum(params.shape[0], grid.shape, device).run(coef).clone()
+
+
+def evaluate_gaussian_input_image(gaussian_rays: GaussianRay, grid, batch_size=128, cutoff: float | None = None):
+    """Complex input-plane field of all beamlets sampled on `grid`."""
+    rays = gaussian_rays
+    assert isinstance(rays, GaussianRay)
+    device = _lib.require_device()
+    params = pack_params(rays.to_vector(), device)
+    coef = beamlet_coefficients(params, grid, MODE_INPUT, None, 0, cutoff)
+    return BeamletSum(params.shape[0], grid.shape, device).run(coef).clone()
