@@ -11,6 +11,9 @@ Prints ONE JSON line (rank 0).  Workloads:
             sample values.
   plain     1,048,576 f32 rays through Lens(0.5, f=1) -> Detector(1.0, 128x128):
             fused run_to_image (positions, pathlength and int32 histogram).
+  beamlet   Gaussian-beamlet aperture diffraction: 131,072 beamlets (300 keV,
+            waist 50 nm, 10 um aperture) propagated 0.3 m onto 1024^2 pixels
+            with the 4-sigma cut; complex64 field.
 
 A "step" is one pass of the hot path over one batch whose inputs are already
 resident in HBM.  Timing: W untimed warm-up steps, then K steps bracketed by
@@ -288,7 +291,106 @@ def bench_stem_fwd(steps: int, warmup: int):
     )
 
 
-WORKLOADS = {"stem_fit": bench_stem_fit, "stem_fwd": bench_stem_fwd, "plain": bench_plain}
+def aperture_beamlets(n=131072, seed=4):
+    """Seeded beamlet centres uniform in a 10 um diameter disc (radius 5 um)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    r = 5e-6 * np.sqrt(rng.uniform(0, 1, n))
+    a = rng.uniform(0, 2 * np.pi, n)
+    return r * np.cos(a), r * np.sin(a)
+
+
+def _ellipse_pixel_count(coef_bytes, n, H, W):
+    """Exact number of (beamlet, pixel) pairs inside the cutoff ellipses."""
+    import numpy as np
+
+    rec = np.frombuffer(coef_bytes, dtype=np.dtype([("phi", "<f8", 12), ("k", "<f4", 2), ("c", "<f4", 2),
+                                                    ("abg", "<f4", 3), ("cut", "<f4"), ("box", "<f4", 2)]))
+    cx, cy = rec["c"][:, 0].astype(np.float64), rec["c"][:, 1].astype(np.float64)
+    al, be, ga = (rec["abg"][:, i].astype(np.float64) for i in range(3))
+    total = 0
+    rows = np.arange(H, dtype=np.float64)
+    for lo in range(0, n, 4096):
+        sl = slice(lo, min(n, lo + 4096))
+        v = rows[None, :] - cy[sl, None]
+        t2 = be[sl, None] - ga[sl, None] * v * v
+        ok = t2 >= 0
+        sq = np.sqrt(np.where(ok, t2, 0.0))
+        mid = cx[sl, None] - al[sl, None] * v
+        lo_px = np.clip(np.ceil(mid - sq), 0, W)
+        hi_px = np.clip(np.floor(mid + sq), -1, W - 1)
+        total += int(np.where(ok, np.maximum(hi_px - lo_px + 1, 0), 0).sum())
+    return total
+
+
+def bench_beamlet(steps: int, warmup: int):
+    import numpy as np
+    import torch
+
+    from oracle import beamlet_oracle as BO
+    from temgymcore_amd.components import Detector
+    from temgymcore_amd.gaussian import (MODE_PROPAGATED, BeamletSum, GaussianRay, _abcd_per_beamlet,
+                                         beamlet_coefficients, pack_params)
+
+    dev = torch.device("cuda", 0)
+    n, H, W, pix, L, lam = 131072, 1024, 1024, 4e-8, 0.3, 1.97e-12
+    x, y = aperture_beamlets(n)
+    z = np.zeros(n)
+    rays = GaussianRay(x=torch.from_numpy(x), y=torch.from_numpy(y), dx=torch.from_numpy(z), dy=torch.from_numpy(z),
+                       z=torch.from_numpy(z), pathlength=torch.from_numpy(z), _one=torch.ones(n, dtype=torch.float64),
+                       amplitude=torch.ones(n, dtype=torch.float64), waist_xy=torch.full((n, 2), 5e-8, dtype=torch.float64),
+                       radii_of_curv=torch.full((n, 2), float("inf"), dtype=torch.float64),
+                       wavelength=torch.full((n,), lam, dtype=torch.float64), theta=torch.from_numpy(z))
+    det = Detector(z=L, pixel_size=(pix, pix), shape=(H, W))
+    params = pack_params(rays, dev)
+    abcd, stride = _abcd_per_beamlet(rays, [det], dev, n)
+    summer = BeamletSum(n, (H, W), dev)
+    holder = {}
+
+    def step():
+        coef = beamlet_coefficients(params, det, MODE_PROPAGATED, abcd, stride, cutoff=4.0)
+        holder["coef"] = coef
+        summer.run(coef)
+
+    ms_per_step, kern_ms, graph_ms = _time_steps(step, steps, warmup)
+    evaluated = _ellipse_pixel_count(holder["coef"].cpu().numpy().tobytes(), n, H, W)
+    dense = n * H * W
+    evals_per_s = evaluated / (ms_per_step * 1e-3)
+    flops_per_eval = 14  # two complex f32 products (6 each) + complex accumulate (2)
+    achieved = flops_per_eval * evals_per_s
+    cores, cpu_model = _cpu_info()
+    cpu = None
+    if not SKIP_CPU:
+        nb, reps, c0 = 16, 0, time.perf_counter()
+        r2 = BO.grid_coords((pix, pix), (H, W))
+        A = np.eye(5)
+        A[0, 2] = A[1, 3] = L
+        d = dict(dx=np.zeros(nb), dy=np.zeros(nb), pathlength=np.zeros(nb), amplitude=np.ones(nb),
+                 waist_xy=np.full((nb, 2), 5e-8), radii=np.full((nb, 2), np.inf), wavelength=np.full(nb, lam),
+                 theta=np.zeros(nb))
+        while time.perf_counter() - c0 < 10.0:
+            BO.make_image(x[:nb], y[:nb], abcd=A, r2=r2, **d)
+            reps += 1
+        cpu_rate = reps * nb * H * W / (time.perf_counter() - c0)
+        cpu = {"value": cpu_rate, "unit": "beamlet-pixel evals/s (dense)", "cores": 1, "kind": "port",
+               "sample": f"numpy complex128 oracle (reference formulas), {nb} beamlets x full 1024^2 detector, "
+                         f"{reps} reps ({cpu_model})"}
+    return dict(
+        metric="beamlet-pixel evals/s (131,072-beamlet aperture diffraction, 1024^2 detector, 4-sigma cut)",
+        value=evals_per_s, unit="beamlet-pixel evals/s", ms_per_step=ms_per_step, dtype="f32 (f64 phase)",
+        config={"workload": "beamlet: 131072 beamlets, 10 um aperture, w0 50 nm, 0.3 m, 1024^2 px of 40 nm",
+                "evaluated_pairs": evaluated, "dense_pairs": dense, "dense_equiv_evals_per_s": dense / (ms_per_step * 1e-3),
+                "event_ms_per_step": kern_ms, "graph_replay_ms_per_step": graph_ms,
+                "kernels_per_step": "beamlet_coef_kernel + beamlet_sum_kernel + split_sum_kernel"},
+        roofline={"bound": "valu", "achieved": achieved / 1e12, "peak": 157.3, "unit": "TFLOP/s",
+                  "frac": achieved / 157.3e12, "traffic": None,
+                  "flops_per_eval": flops_per_eval},
+        cpu_baseline=cpu,
+    )
+
+
+WORKLOADS = {"stem_fit": bench_stem_fit, "stem_fwd": bench_stem_fwd, "plain": bench_plain, "beamlet": bench_beamlet}
 DEFAULT = "stem_fit"
 
 

@@ -211,8 +211,11 @@ typedef struct tg_beamlet_coef {
                           Phi = a px^2 + b px py + c py^2 + d px + e py + f      */
   float k_re, k_im;    /* exp(2a): ratio of successive x-step ratios             */
   float cx, cy;        /* envelope centre (pixels)                               */
-  float s00, s01, s11; /* envelope form S: |field| = peak * exp(-(p-c)^T S (p-c)) */
-  float cut;           /* contributions kept where (p-c)^T S (p-c) <= cut        */
+  /* kept pixels of row py (v = py - cy): cx - alpha v +- sqrt(beta - gamma v^2)
+   * (gamma <= 0: the whole row); the cutoff ellipse is
+   * (p-c)^T S (p-c) <= cut with |field| = peak * exp(-(p-c)^T S (p-c)) */
+  float alpha, beta, gamma;
+  float cut;
   float box_w, box_h;  /* half-extents of that ellipse (pixels; huge: unbounded) */
 } tg_beamlet_coef;     /* 144 bytes */
 

@@ -202,15 +202,16 @@ __global__ __launch_bounds__(256) void beamlet_coef_kernel(const double* __restr
     const double g0 = cdd.re, g1 = ce.re;
     o.cx = (float)((s11 * g0 - s01 * g1) / (2.0 * det));
     o.cy = (float)((s00 * g1 - s01 * g0) / (2.0 * det));
-    o.s00 = (float)s00;
-    o.s01 = (float)s01;
-    o.s11 = (float)s11;
+    // s00 u^2 + 2 s01 u v + s11 v^2 <= cut  <=>  u = -alpha v +- sqrt(beta - gamma v^2)
+    o.alpha = (float)(s01 / s00);
+    o.beta = (float)((double)cut / s00);
+    o.gamma = (float)(det / (s00 * s00));
     o.cut = cut;
     o.box_w = (float)sqrt((double)cut * s11 / det);
     o.box_h = (float)sqrt((double)cut * s00 / det);
   } else {  // no decaying envelope: every pixel
     o.cx = o.cy = 0.0f;
-    o.s00 = o.s01 = o.s11 = 0.0f;
+    o.alpha = o.beta = o.gamma = 0.0f;
     o.cut = cut;
     o.box_w = o.box_h = kUnbounded;
   }
@@ -286,13 +287,13 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_kernel(const tg_beamlet_
       // this lane's pixels inside the cutoff ellipse: [xs, xe]
       int64_t xs = x0, xe = x0 + kRun - 1;
       if (xe >= W) xe = W - 1;
-      if (c.s00 > 0.0f) {
+      if (c.gamma > 0.0f) {
         const float v = (float)row - c.cy;
-        const float disc = c.s01 * c.s01 * v * v - c.s00 * (c.s11 * v * v - c.cut);
-        if (disc < 0.0f) continue;
-        const float sq = sqrtf(disc);
-        const float ul = (-c.s01 * v - sq) / c.s00, ur = (-c.s01 * v + sq) / c.s00;
-        const float lo = ceilf(c.cx + ul), hi = floorf(c.cx + ur);
+        const float t2 = fmaf(-c.gamma * v, v, c.beta);
+        if (t2 < 0.0f) continue;
+        const float sq = sqrtf(t2);
+        const float mid = fmaf(-c.alpha, v, c.cx);
+        const float lo = ceilf(mid - sq), hi = floorf(mid + sq);
         if (lo > (float)xs) xs = (lo > (float)(x0 + kRun)) ? x0 + kRun : (int64_t)lo;
         if (hi < (float)xe) xe = (hi < (float)(x0 - 1)) ? x0 - 1 : (int64_t)hi;
       }
@@ -343,11 +344,14 @@ __global__ __launch_bounds__(256) void split_sum_kernel(const float2* __restrict
 
 int splits_for(int64_t n, int64_t H, int64_t W) {
   const int64_t ntiles = tg::cdiv(H, kTile) * tg::cdiv(W, kTile);
-  const int64_t want = 8LL * tg::cu_count();
+  // Many small work items (tile x beamlet slice) let the hardware scheduler
+  // balance tiles near the centre of the pattern, which overlap many more
+  // beamlets than the edge tiles.
+  const int64_t want = 64LL * tg::cu_count();
   int64_t s = tg::cdiv(want, ntiles);
-  const int64_t by_n = tg::cdiv(n, 64);  // at least ~64 beamlets per slice
+  const int64_t by_n = tg::cdiv(n, 256);  // at least ~256 beamlets per slice
   if (s > by_n) s = by_n;
-  if (s > 64) s = 64;
+  if (s > 128) s = 128;
   if (s < 1) s = 1;
   return (int)s;
 }

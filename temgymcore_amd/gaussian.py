@@ -32,7 +32,7 @@ class BeamletCoef(ctypes.Structure):
         ("phi", ctypes.c_double * 12),
         ("k_re", ctypes.c_float), ("k_im", ctypes.c_float),
         ("cx", ctypes.c_float), ("cy", ctypes.c_float),
-        ("s00", ctypes.c_float), ("s01", ctypes.c_float), ("s11", ctypes.c_float),
+        ("alpha", ctypes.c_float), ("beta", ctypes.c_float), ("gamma", ctypes.c_float),
         ("cut", ctypes.c_float),
         ("box_w", ctypes.c_float), ("box_h", ctypes.c_float),
     ]
