@@ -221,9 +221,11 @@ __global__ __launch_bounds__(256) void beamlet_coef_kernel(const double* __restr
 // ---------------------------------------------------------------------------
 // stage 2: tile sums
 // ---------------------------------------------------------------------------
-constexpr int kTile = 64;               // tile is kTile x kTile pixels
-constexpr int kRun = 16;                // pixels per lane (one row segment)
-constexpr int kThreads = kTile * kTile / kRun;  // 256
+constexpr int kRun = 16;                  // pixels per lane (one row segment)
+constexpr int kLanesPerRow = 4;
+constexpr int kTileW = kRun * kLanesPerRow;  // tile is kTileW x kTileH pixels
+constexpr int kTileH = 64;
+constexpr int kThreads = kLanesPerRow * kTileH;  // 256
 constexpr int kWaves = kThreads / 64;
 
 // (x - rint(x)) of x * (1 / 2 pi) in double-double: the phase in revolutions,
@@ -252,15 +254,15 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_kernel(const tg_beamlet_
   __shared__ int32_t list[kThreads];
   __shared__ int32_t wave_cnt[kWaves];
   const int tile = blockIdx.x % ntiles, split = blockIdx.x / ntiles;
-  const int64_t tx0 = (int64_t)(tile % ntx) * kTile, ty0 = (int64_t)(tile / ntx) * kTile;
+  const int64_t tx0 = (int64_t)(tile % ntx) * kTileW, ty0 = (int64_t)(tile / ntx) * kTileH;
   const int64_t b_begin = n * split / nsplit, b_end = n * (split + 1) / nsplit;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int64_t row = ty0 + t / (kTile / kRun);
-  const int64_t x0 = tx0 + (int64_t)(t % (kTile / kRun)) * kRun;
+  const int64_t row = ty0 + t / kLanesPerRow;
+  const int64_t x0 = tx0 + (int64_t)(t % kLanesPerRow) * kRun;
   float2 acc[kRun];
 #pragma unroll
   for (int j = 0; j < kRun; ++j) acc[j] = make_float2(0.f, 0.f);
-  const float ftx0 = (float)tx0, fty0 = (float)ty0, ftx1 = (float)(tx0 + kTile - 1), fty1 = (float)(ty0 + kTile - 1);
+  const float ftx0 = (float)tx0, fty0 = (float)ty0, ftx1 = (float)(tx0 + kTileW - 1), fty1 = (float)(ty0 + kTileH - 1);
   for (int64_t b0 = b_begin; b0 < b_end; b0 += kThreads) {
     // compact the beamlets of this chunk whose cutoff ellipse meets the tile
     const int64_t b = b0 + t;
@@ -343,7 +345,7 @@ __global__ __launch_bounds__(256) void split_sum_kernel(const float2* __restrict
 }
 
 int splits_for(int64_t n, int64_t H, int64_t W) {
-  const int64_t ntiles = tg::cdiv(H, kTile) * tg::cdiv(W, kTile);
+  const int64_t ntiles = tg::cdiv(H, kTileH) * tg::cdiv(W, kTileW);
   // Many small work items (tile x beamlet slice) let the hardware scheduler
   // balance tiles near the centre of the pattern, which overlap many more
   // beamlets than the edge tiles.
@@ -392,7 +394,7 @@ extern "C" int tg_beamlet_sum(const tg_beamlet_coef* coef, int64_t n, int64_t he
   const int64_t need = (int64_t)nsplit * height * width * (int64_t)sizeof(float2);
   TG_REQUIRE(workspace_bytes >= need, "tg_beamlet_sum: workspace %lld bytes < %lld", (long long)workspace_bytes,
              (long long)need);
-  const int64_t ntx = tg::cdiv(width, kTile), nty = tg::cdiv(height, kTile);
+  const int64_t ntx = tg::cdiv(width, kTileW), nty = tg::cdiv(height, kTileH);
   const int64_t ntiles = ntx * nty;
   TG_REQUIRE(ntiles * nsplit < (int64_t)1 << 31, "tg_beamlet_sum: detector too large");
   hipLaunchKernelGGL(beamlet_sum_kernel, dim3((unsigned)(ntiles * nsplit)), dim3(kThreads), 0, (hipStream_t)stream,
