@@ -14,6 +14,10 @@ Prints ONE JSON line (rank 0).  Workloads:
   beamlet   Gaussian-beamlet aperture diffraction: 131,072 beamlets (300 keV,
             waist 50 nm, 10 um aperture) propagated 0.3 m onto 1024^2 pixels
             with the 4-sigma cut; complex64 field.
+  gauss_fit batched Gaussian-decomposition fits, forward + adjoint: 256
+            instances with 3-2,000 Gaussians (log-uniform, seed 5) on square
+            detectors of 128-512 pixels (seed 6), 1 um pixels, 2.51 pm;
+            per-instance intensity loss and the gradient of every parameter row.
 
 A "step" is one pass of the hot path over one batch whose inputs are already
 resident in HBM.  Timing: W untimed warm-up steps, then K steps bracketed by
@@ -390,7 +394,84 @@ def bench_beamlet(steps: int, warmup: int):
     )
 
 
-WORKLOADS = {"stem_fit": bench_stem_fit, "stem_fwd": bench_stem_fwd, "plain": bench_plain, "beamlet": bench_beamlet}
+def bench_gauss_fit(steps: int, warmup: int):
+    import numpy as np
+    import torch
+
+    from oracle import beamlet_oracle as BO
+    from oracle import gauss_fit_oracle as GF
+    from temgymcore_amd import gauss_fit
+
+    batch, params, target = gauss_fit.workload()
+    dev = batch.device
+    loss = torch.empty(batch.n_inst, dtype=torch.float64, device=dev)
+    grad = torch.empty((batch.n_gauss, 16), dtype=torch.float64, device=dev)
+
+    def step():
+        batch.run(params, target, None, loss, grad)
+
+    def fwd():
+        batch.run(params, target, None, loss, None)
+
+    ms_per_step, kern_ms, graph_ms = _time_steps(step, steps, warmup)
+    fwd_ms, fwd_kern_ms, _ = _time_steps(fwd, max(steps // 2, 1), 1)
+    torch.cuda.synchronize()
+    assert torch.isfinite(loss).all() and torch.isfinite(grad).all()
+    # pixels inside each Gaussian's cutoff ellipse, clipped to its detector
+    rec = batch.workspace[batch.plan.off_coef:batch.plan.off_coef + 136 * batch.n_gauss].cpu().numpy().tobytes()
+    evaluated = 0
+    for i in range(batch.n_inst):
+        g0, g1 = int(batch.gauss_offsets[i]), int(batch.gauss_offsets[i + 1])
+        if g1 > g0:
+            evaluated += _ellipse_pixel_count(rec[136 * g0:136 * g1], g1 - g0, *batch.shapes[i])
+    dense = int(sum(c * h * w for c, (h, w) in zip(batch.counts, batch.shapes)))
+    units = 2 * dense  # forward + backward Gaussian-pixel evaluations, dense count
+    value = units / (ms_per_step * 1e-3)
+    flops_fwd, flops_bwd = 14, 28  # per evaluated pair (recurrence + accumulate; + W product and 3 moments)
+    achieved = (flops_fwd + flops_bwd) * evaluated / (kern_ms * 1e-3)
+    cores, cpu_model = _cpu_info()
+    cpu = None
+    if not SKIP_CPU:
+        # the oracle's loss + gradient over the first (up to) 16 Gaussians of
+        # each instance on its full detector, instances in workload order, ~10 s
+        p = params.cpu().numpy()
+        t = target.cpu().numpy().astype(np.float64)
+        done, c0, i = 0, time.perf_counter(), 0
+        names = []
+        while i < batch.n_inst and (time.perf_counter() - c0 < 10.0 or done == 0):
+            g0 = int(batch.gauss_offsets[i])
+            g1 = min(int(batch.gauss_offsets[i + 1]), g0 + 16)
+            h, w = batch.shapes[i]
+            r1 = BO.grid_coords((1e-6, 1e-6), (h, w))
+            tt = t[int(batch.pix_offsets[i]):int(batch.pix_offsets[i + 1])]
+            GF.loss(p[g0:g1], r1, tt)
+            GF.grad(p[g0:g1], r1, tt)
+            done += 2 * (g1 - g0) * h * w
+            names.append(f"{g1 - g0}x{h}^2")
+            i += 1
+        cpu_rate = done / (time.perf_counter() - c0)
+        cpu = {"value": cpu_rate, "unit": "Gaussian-pixel evals/s fwd+bwd (dense)", "cores": 1, "kind": "port",
+               "sample": f"numpy complex128 oracle loss + analytic gradient, first <=16 Gaussians of instances 0..{i - 1} "
+                         f"({', '.join(names[:6])}{'...' if len(names) > 6 else ''}) ({cpu_model})"}
+    return dict(
+        metric="Gaussian-pixel evals/s fwd+bwd (256 batched Gaussian-decomposition fits)",
+        value=value, unit="Gaussian-pixel evals/s", ms_per_step=ms_per_step, dtype="f32 (f64 phase, reductions)",
+        config={"workload": "gauss_fit: 256 instances, 3-2000 Gaussians (log-uniform, seed 5), "
+                            "128-512 px square detectors (seed 6), 1 um pixels, 2.51 pm, no cutoff",
+                "gaussians": batch.n_gauss, "pixels": batch.n_pix, "dense_pairs": dense,
+                "evaluated_pairs_per_pass": evaluated, "event_ms_per_step": kern_ms,
+                "graph_replay_ms_per_step": graph_ms, "forward_only_event_ms": fwd_kern_ms,
+                "kernels_per_step": "fit_coef, fit_forward, fit_loss, fit_loss_final, fit_moment, fit_grad"},
+        roofline={"bound": "valu", "achieved": achieved / 1e12, "peak": 157.3, "unit": "TFLOP/s",
+                  "frac": achieved / 157.3e12, "traffic": None,
+                  "flops_per_eval": {"forward": flops_fwd, "backward": flops_bwd},
+                  "kernels": "fit_forward_kernel + fit_moment_kernel (whole step timed)"},
+        cpu_baseline=cpu,
+    )
+
+
+WORKLOADS = {"stem_fit": bench_stem_fit, "stem_fwd": bench_stem_fwd, "plain": bench_plain, "beamlet": bench_beamlet,
+             "gauss_fit": bench_gauss_fit}
 DEFAULT = "stem_fit"
 
 

@@ -31,7 +31,7 @@ extern "C" {
 const char* tg_last_error(void);
 
 /* ABI version; bumped whenever a struct layout below changes. */
-int32_t tg_abi_version(void);  /* currently 2 */
+int32_t tg_abi_version(void);  /* currently 3 */
 
 /* ======================================================================== *
  * Ray branch: per-ray register programs
@@ -217,7 +217,7 @@ typedef struct tg_beamlet_coef {
   float alpha, beta, gamma;
   float cut;
   float box_w, box_h;  /* half-extents of that ellipse (pixels; huge: unbounded) */
-} tg_beamlet_coef;     /* 144 bytes */
+} tg_beamlet_coef;     /* 136 bytes */
 
 /* Coefficients of n beamlets.
  *   params  [n, TG_BEAMLET_NPARAM] f64 (device)
@@ -236,6 +236,65 @@ int tg_beamlet_coefficients(const double* params, int64_t n, const double* abcd,
 int64_t tg_beamlet_sum_workspace(int64_t n, int64_t height, int64_t width);
 int tg_beamlet_sum(const tg_beamlet_coef* coef, int64_t n, int64_t height, int64_t width, float* field,
                    void* workspace, int64_t workspace_bytes, void* stream);
+
+/* ======================================================================== *
+ * Batched Gaussian-decomposition fits
+ *
+ * Many independent instances, each a set of input-plane Gaussians
+ * (evaluate_gaussian_input_image, gaussian.py:372-427) sampled on its own
+ * detector grid, with the intensity loss of the reference's fitting example
+ * (examples/fit_three_gaussians.ipynb, cell 2)
+ *     L_i = mean over pixels of (|u_i|^2 - T_i)^2
+ * and its gradient with respect to every parameter row (the jax.grad of
+ * that loss in the example).  Instance sizes may differ; all instances run
+ * in the same launches.
+ * ======================================================================== */
+
+typedef struct tg_fit_instance {
+  int64_t gauss_begin;  /* first parameter row of the instance                 */
+  int64_t n_gauss;      /* its number of Gaussians (0 allowed)                 */
+  int64_t pix_begin;    /* first pixel of its image in the packed pixel arrays */
+  int32_t height, width;
+  double pix2m[6];      /* pixel->metre map, as for tg_beamlet_coefficients    */
+} tg_fit_instance;      /* 80 bytes */
+
+/* Launch geometry and workspace layout, filled in by tg_gauss_fit_plan. */
+typedef struct tg_fit_plan {
+  int32_t n_inst, n_items, n_loss_blocks, n_moment_blocks;
+  int64_t n_gauss;          /* parameter rows                                     */
+  int64_t n_assigned;       /* rows that belong to an instance                    */
+  int64_t n_pix;            /* length of the packed pixel arrays                  */
+  float cut;
+  int32_t reserved;
+  int64_t off_inst, off_items, off_loss_blocks, off_order, off_gauss_inst;
+  int64_t off_coef, off_unit, off_partial, off_weight, off_loss_part, off_moments;
+  int64_t table_bytes;      /* bytes of the tables uploaded by tg_gauss_fit_upload */
+  int64_t workspace_bytes;  /* total device workspace                             */
+} tg_fit_plan;
+
+/* Host-only: checks the instances (Gaussian rows inside [0, n_gauss), pixel
+ * and Gaussian ranges pairwise disjoint) and lays out the workspace.  nsigma
+ * as for tg_beamlet_coefficients.  No reference counterpart (the reference
+ * fits one instance at a time). */
+int tg_gauss_fit_plan(const tg_fit_instance* inst, int32_t n_inst, int64_t n_gauss, double nsigma,
+                      tg_fit_plan* plan);
+
+/* Copies the plan's tables (built from the same host `inst` array) into the
+ * device workspace of plan->workspace_bytes and waits for the copy. */
+int tg_gauss_fit_upload(const tg_fit_plan* plan, const tg_fit_instance* inst, void* workspace, void* stream);
+
+/* One pass of every instance, stream-ordered, no allocation, no host sync:
+ *   params  [n_gauss, TG_BEAMLET_NPARAM] f64 parameter rows
+ *   target  [n_pix] f32 target intensities |u_target|^2, or NULL
+ *   field   [n_pix] complex64 (interleaved f32) fields, or NULL
+ *   loss    [n_inst] f64 losses (requires target), or NULL
+ *   grad    [n_gauss, TG_BEAMLET_NPARAM] f64 d(sum of losses)/d(params)
+ *           (requires target; columns 13-15 and rows of no instance are 0),
+ *           or NULL
+ * Replaces evaluate_gaussian_input_image plus the jax.grad of the example's
+ * loss (gaussian.py:372-427; fit_three_gaussians.ipynb cells 2-3). */
+int tg_gauss_fit_run(const tg_fit_plan* plan, void* workspace, const double* params, const float* target,
+                     float* field, double* loss, double* grad, void* stream);
 
 /* ======================================================================== *
  * Measurement support

@@ -1,0 +1,268 @@
+"""GPU parity of the batched Gaussian-fit kernels (forward field, intensity
+loss, hand-written adjoint) against the complex128 oracle, whose gradient is
+pinned by finite differences in test_oracle_gauss_fit.py.
+
+Tolerances (f32 fields and moments, f64 reductions):
+  field  |u - ref| <= 2e-5 max|ref|, phase <= 1e-4 rad where |ref| > 1e-3 max|ref|
+  loss   relative 1e-4
+  grad   every entry |g - ref| <= 1e-4 S, where S = sum over pixels of
+         |W du_g/dtheta| is the size of the terms the entry sums (an f32 sum
+         of terms of size S carries an error of order 1e-7 S whatever its
+         result; entries that are small remainders of large cancelling sums
+         are held to that, not to their own size), and per column
+         |g - ref| <= 1e-2 max_g |ref[:, column]| for columns whose largest
+         entry is not itself a cancellation remainder.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import beamlet_oracle as BO
+from oracle import gauss_fit_oracle as GF
+from temgymcore_amd import gauss_fit
+from temgymcore_amd.components import Detector
+from temgymcore_amd.gaussian import GaussianRay, evaluate_gaussian_input_image
+
+pytestmark = pytest.mark.gpu
+
+PIX = 1e-6
+LAM = 2.51e-12
+
+
+def _field_ok(got, ref, amp_tol=2e-5, phase_tol=1e-4):
+    got = got.astype(np.complex128)
+    scale = np.abs(ref).max()
+    if scale == 0:
+        assert np.abs(got).max() == 0
+        return
+    err = np.abs(got - ref)
+    assert err.max() <= amp_tol * scale, f"max |d| {err.max() / scale:.3e} of max|ref|"
+    strong = np.abs(ref) > 1e-3 * scale
+    dphi = np.angle(got[strong] / ref[strong])
+    assert np.abs(dphi).max() <= phase_tol, f"max phase error {np.abs(dphi).max():.3e} rad"
+
+
+def _grad_ok(got, ref, mag, tol=1e-4, col_tol=1e-2):
+    if ref.shape[0] == 0:
+        return
+    err = np.abs(got[:, :13] - ref[:, :13])
+    bad = err > tol * mag[:, :13]
+    assert not bad.any(), f"entries {np.argwhere(bad)}: err / S {(err / np.maximum(mag[:, :13], 1e-300))[bad]}"
+    # column-level check where a column is not all cancellation (e.g. phase-only
+    # parameters of Gaussians that do not overlap have a zero exact gradient)
+    scale = np.abs(ref[:, :13]).max(axis=0)
+    live = scale > 1e-2 * mag[:, :13].max(axis=0)
+    colbad = live & (err.max(axis=0) > col_tol * scale)
+    assert not colbad.any(), f"columns {np.nonzero(colbad)[0]}: {(err.max(axis=0) / scale)[colbad]}"
+    assert np.all(got[:, 13:] == 0)
+
+
+def _params(n, side_hw, rng, tilt=1e-7, finite_radius=False):
+    H, W = side_hw
+    p = np.zeros((n, 16))
+    p[:, 0] = rng.uniform(-0.4, 0.4, n) * W * PIX
+    p[:, 1] = rng.uniform(-0.4, 0.4, n) * H * PIX
+    p[:, 2:4] = rng.uniform(-tilt, tilt, (n, 2))
+    p[:, 4] = rng.uniform(0, 1e-12, n)
+    amp = rng.uniform(0.3, 1.0, n) * np.exp(1j * rng.uniform(-np.pi, np.pi, n))
+    p[:, 5], p[:, 6] = amp.real, amp.imag
+    p[:, 7:9] = rng.uniform(3e-6, 12e-6, (n, 2))
+    p[:, 9:11] = np.inf
+    if finite_radius and n:
+        p[0, 9:11] = (60.0, -90.0)
+    p[:, 11] = LAM
+    p[:, 12] = rng.uniform(-np.pi / 2, np.pi / 2, n)
+    return p
+
+
+def _batch(shapes, counts, cutoff=None):
+    grids = [Detector(z=0.0, pixel_size=(PIX, PIX), shape=s) for s in shapes]
+    return gauss_fit.GaussianFitBatch(grids, counts, cutoff), grids
+
+
+def _oracle_instance(batch, p_np, target_np, i):
+    H, W = batch.shapes[i]
+    r1 = BO.grid_coords((PIX, PIX), (H, W))
+    g0, g1 = batch.gauss_offsets[i], batch.gauss_offsets[i + 1]
+    t = target_np[batch.pix_offsets[i]:batch.pix_offsets[i + 1]].astype(np.float64)
+    p = p_np[g0:g1]
+    return GF.field(p, r1), GF.loss(p, r1, t), GF.grad(p, r1, t), GF.grad(p, r1, t, magnitude=True)
+
+
+def _ragged_case(seed=0, tilt=1e-7):
+    rng = np.random.default_rng(seed)
+    shapes = [(40, 48), (70, 33), (130, 97), (64, 64), (1, 1), (65, 129)]
+    counts = [3, 0, 17, 5, 2, 40]
+    batch, _ = _batch(shapes, counts)
+    p = np.concatenate([_params(n, s, rng, tilt=tilt, finite_radius=True) for n, s in zip(counts, shapes)])
+    tgt = p.copy()
+    tgt[:, 0:2] += rng.normal(0, PIX, (len(p), 2))
+    tgt[:, 7:9] *= 1 + rng.normal(0, 0.05, (len(p), 2))
+    target = np.concatenate([
+        np.abs(GF.field(tgt[batch.gauss_offsets[i]:batch.gauss_offsets[i + 1]],
+                        BO.grid_coords((PIX, PIX), s))) ** 2 for i, s in enumerate(shapes)]).astype(np.float32)
+    return batch, p, target
+
+
+@pytest.mark.parametrize("tilt", [1e-7, 2e-3])
+def test_ragged_batch_matches_oracle(device, tilt):
+    batch, p, target = _ragged_case(tilt=tilt)
+    params = torch.from_numpy(p).to(device)
+    tg = torch.from_numpy(target).to(device)
+    field = torch.empty(batch.n_pix, dtype=torch.complex64, device=device)
+    loss = torch.empty(batch.n_inst, dtype=torch.float64, device=device)
+    grad = torch.empty((batch.n_gauss, 16), dtype=torch.float64, device=device)
+    batch.run(params, target=tg, field=field, loss=loss, grad=grad)
+    torch.cuda.synchronize()
+    f, l, g = field.cpu().numpy(), loss.cpu().numpy(), grad.cpu().numpy()
+    for i in range(batch.n_inst):
+        u, L, G, S = _oracle_instance(batch, p, target, i)
+        print(f"instance {i} shape {batch.shapes[i]} n {batch.counts[i]}", flush=True)
+        _field_ok(f[batch.pix_offsets[i]:batch.pix_offsets[i + 1]], u)
+        assert l[i] == pytest.approx(L, rel=1e-4, abs=1e-12)
+        _grad_ok(g[batch.gauss_offsets[i]:batch.gauss_offsets[i + 1]], G, S)
+
+
+def test_fit_loss_autograd_weights_rows_by_instance(device):
+    batch, p, target = _ragged_case(seed=1)
+    params = torch.from_numpy(p).to(device).requires_grad_(True)
+    tg = torch.from_numpy(target).to(device)
+    loss = gauss_fit.fit_loss(params, batch, tg)
+    w = torch.arange(1, batch.n_inst + 1, dtype=torch.float64, device=device)
+    (loss * w).sum().backward()
+    _, raw = batch.loss_and_grad(params.detach(), tg)
+    wi = w[batch.gauss_instance]
+    torch.testing.assert_close(params.grad, raw * wi[:, None], rtol=0, atol=0)
+
+
+def test_loss_and_grad_vanish_at_own_intensity(device):
+    rng = np.random.default_rng(2)
+    shapes, counts = [(90, 77), (64, 128)], [6, 9]
+    batch, _ = _batch(shapes, counts)
+    p = torch.from_numpy(np.concatenate([_params(n, s, rng) for n, s in zip(counts, shapes)])).to(device)
+    target = batch.intensity(p)
+    loss, grad = batch.loss_and_grad(p, target)
+    # moved targets give the scale the vanishing values are compared with
+    p2 = p.clone()
+    p2[:, 0] += 2 * PIX
+    loss2, grad2 = batch.loss_and_grad(p2, target)
+    assert torch.all(loss <= 1e-10 * loss2)
+    assert grad.abs().max() <= 1e-4 * grad2.abs().max()
+
+
+def test_batch_matches_single_image_path(device):
+    rng = np.random.default_rng(3)
+    shapes, counts = [(100, 90), (70, 140)], [12, 30]
+    batch, grids = _batch(shapes, counts)
+    p = np.concatenate([_params(n, s, rng, tilt=1e-3) for n, s in zip(counts, shapes)])
+    imgs = batch.images(batch.field(torch.from_numpy(p).to(device)))
+    for i, grid in enumerate(grids):
+        q = p[batch.gauss_offsets[i]:batch.gauss_offsets[i + 1]]
+        n = q.shape[0]
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a))  # noqa: E731
+        rays = GaussianRay(x=t(q[:, 0]), y=t(q[:, 1]), dx=t(q[:, 2]), dy=t(q[:, 3]), z=torch.zeros(n, dtype=torch.float64),
+                           pathlength=t(q[:, 4]), _one=torch.ones(n, dtype=torch.float64),
+                           amplitude=t(q[:, 5] + 1j * q[:, 6]), waist_xy=t(q[:, 7:9]), radii_of_curv=t(q[:, 9:11]),
+                           wavelength=t(q[:, 11]), theta=t(q[:, 12]))
+        single = evaluate_gaussian_input_image(rays, grid)
+        ref = single.cpu().numpy().astype(np.complex128)
+        got = imgs[i].cpu().numpy().astype(np.complex128)
+        assert np.abs(got - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+def test_instance_order_does_not_change_results(device):
+    """Each instance's results do not depend on the rest of the batch (bitwise)."""
+    rng = np.random.default_rng(4)
+    shapes, counts = [(80, 70), (50, 200), (33, 33)], [300, 4, 11]
+    pp = [_params(n, s, rng) for n, s in zip(counts, shapes)]
+    tt = [rng.uniform(0, 1, s[0] * s[1]).astype(np.float32) for s in shapes]
+    res = {}
+    for order in ([0, 1, 2], [2, 0, 1]):
+        batch, _ = _batch([shapes[k] for k in order], [counts[k] for k in order])
+        params = torch.from_numpy(np.concatenate([pp[k] for k in order])).to(device)
+        target = torch.from_numpy(np.concatenate([tt[k] for k in order])).to(device)
+        field = torch.empty(batch.n_pix, dtype=torch.complex64, device=device)
+        loss = torch.empty(batch.n_inst, dtype=torch.float64, device=device)
+        grad = torch.empty((batch.n_gauss, 16), dtype=torch.float64, device=device)
+        batch.run(params, target, field, loss, grad)
+        for j, k in enumerate(order):
+            res.setdefault(k, []).append((
+                field[batch.pix_offsets[j]:batch.pix_offsets[j + 1]].cpu(), loss[j].item(),
+                grad[batch.gauss_offsets[j]:batch.gauss_offsets[j + 1]].cpu()))
+    for k, (a, b) in res.items():
+        assert torch.equal(a[0], b[0]) and a[1] == b[1] and torch.equal(a[2], b[2]), k
+
+
+def test_cutoff_and_far_gaussians(device):
+    rng = np.random.default_rng(5)
+    shapes, counts = [(96, 96)], [8]
+    p = _params(8, shapes[0], rng)
+    p[0, 0] = 1.0  # a metre away: contributes nothing, gradient zero
+    target = np.abs(GF.field(p[1:], BO.grid_coords((PIX, PIX), shapes[0]))) ** 2 * 1.1
+    for cutoff in (None, 4.0):
+        batch, _ = _batch(shapes, counts, cutoff)
+        loss, grad = batch.loss_and_grad(torch.from_numpy(p).to(device),
+                                         torch.from_numpy(target.astype(np.float32)).to(device))
+        g = grad.cpu().numpy()
+        assert np.all(np.isfinite(g)) and np.all(g[0] == 0)
+        _, L, G, S = _oracle_instance(batch, p, target.astype(np.float32), 0)
+        # the 4-sigma cut drops tails below exp(-16) of each peak
+        assert loss.item() == pytest.approx(L, rel=1e-4 if cutoff is None else 1e-3)
+        if cutoff is None:
+            _grad_ok(g, G, S)
+        else:
+            _grad_ok(g, G, S, tol=1e-2, col_tol=2e-2)
+
+
+def test_graph_replay_matches_eager(device):
+    batch, p, target = _ragged_case(seed=6)
+    params = torch.from_numpy(p).to(device)
+    tg = torch.from_numpy(target).to(device)
+    loss = torch.empty(batch.n_inst, dtype=torch.float64, device=device)
+    grad = torch.empty((batch.n_gauss, 16), dtype=torch.float64, device=device)
+    batch.run(params, tg, None, loss, grad)
+    ref_l, ref_g = loss.clone(), grad.clone()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        batch.run(params, tg, None, loss, grad)
+    torch.cuda.current_stream().wait_stream(s)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        batch.run(params, tg, None, loss, grad)
+    for _ in range(3):
+        loss.zero_()
+        grad.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(loss, ref_l) and torch.equal(grad, ref_g)
+
+
+def test_rejects_bad_arguments(device):
+    batch, p, target = _ragged_case(seed=7)
+    params = torch.from_numpy(p).to(device)
+    with pytest.raises(ValueError):
+        batch.run(params.float())
+    with pytest.raises(ValueError):
+        batch.run(params, loss=torch.empty(batch.n_inst, dtype=torch.float64, device=device))
+
+
+def test_full_workload_properties_and_sampled_instances(device):
+    """The 256-instance workload: finite results, and the instances with the
+    fewest Gaussians against the oracle at full size."""
+    batch, params, target = gauss_fit.workload()
+    loss, grad = batch.loss_and_grad(params, target)
+    torch.cuda.synchronize()
+    assert batch.n_inst == 256 and torch.isfinite(loss).all() and torch.isfinite(grad).all()
+    assert (loss > 0).all()
+    p = params.cpu().numpy()
+    t = target.cpu().numpy()
+    l, g = loss.cpu().numpy(), grad.cpu().numpy()
+    field = batch.field(params).cpu().numpy()
+    for i in np.argsort(np.asarray(batch.counts), kind="stable")[:3]:
+        print(f"instance {i} shape {batch.shapes[i]} n {batch.counts[i]}", flush=True)
+        u, L, G, S = _oracle_instance(batch, p, t, i)
+        _field_ok(field[batch.pix_offsets[i]:batch.pix_offsets[i + 1]], u)
+        assert l[i] == pytest.approx(L, rel=1e-4)
+        _grad_ok(g[batch.gauss_offsets[i]:batch.gauss_offsets[i + 1]], G, S)

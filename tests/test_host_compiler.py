@@ -247,6 +247,8 @@ def _declared_symbols():
 def test_library_exports_every_declared_symbol():
     if not os.path.exists(_lib.LIB_PATH):
         pytest.skip("library not built")
+    from temgymcore_amd import gauss_fit, gaussian, stem  # noqa: F401  (register their signatures)
+
     lib = ctypes.CDLL(_lib.LIB_PATH)
     names = _declared_symbols()
     assert "tg_ray_program_run" in names
@@ -260,17 +262,27 @@ def test_struct_layout_matches_header(tmp_path):
     import shutil
     import subprocess
 
+    from temgymcore_amd import gauss_fit, gaussian, stem
+
     cc = shutil.which("gcc") or shutil.which("cc")
     if cc is None:
         pytest.skip("no C compiler")
+    mirrors = {
+        "tg_instr": _lib.Instr, "tg_slot": _lib.Slot, "tg_program": _lib.Program, "tg_bin_spec": _lib.BinSpec,
+        "tg_stem_geometry": stem.StemGeometry, "tg_beamlet_coef": gaussian.BeamletCoef,
+        "tg_fit_instance": gauss_fit.FitInstance, "tg_fit_plan": gauss_fit.FitPlan,
+    }
+    lines, expect = [], []
+    for cname, py in mirrors.items():
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        expect.append(ctypes.sizeof(py))
+        for fname, *_ in py._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {fname}));')
+            expect.append(getattr(py, fname).offset)
     src = tmp_path / "sz.c"
-    src.write_text(
-        '#include <stdio.h>\n#include <stddef.h>\n#include "temgym_hip.h"\n'
-        'int main(){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(tg_instr), sizeof(tg_slot),'
-        ' sizeof(tg_program), sizeof(tg_bin_spec), offsetof(tg_program, out_dtype), offsetof(tg_bin_spec, acc));}'
-    )
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "temgym_hip.h"\nint main(){'
+                   + "".join(lines) + "}")
     exe = tmp_path / "sz"
     subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    assert got == [ctypes.sizeof(_lib.Instr), ctypes.sizeof(_lib.Slot), ctypes.sizeof(_lib.Program),
-                   ctypes.sizeof(_lib.BinSpec), _lib.Program.out_dtype.offset, _lib.BinSpec.acc.offset]
+    assert got == expect
