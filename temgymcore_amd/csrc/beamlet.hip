@@ -44,15 +44,15 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_kernel(const tg_beamlet_
                                                                int32_t nsplit, float2* __restrict__ ws) {
   __shared__ TileLists sh;
   const int tile = blockIdx.x % ntiles, split = blockIdx.x / ntiles;
-  const int64_t tx0 = (int64_t)(tile % ntx) * kTileW, ty0 = (int64_t)(tile / ntx) * kTileH;
+  const int32_t tx0 = (tile % ntx) * kTileW, ty0 = (tile / ntx) * kTileH;
   const int64_t b_begin = n * split / nsplit, b_end = n * (split + 1) / nsplit;
   const int t = threadIdx.x;
-  const int64_t row = ty0 + t / kLanesPerRow;
-  const int64_t x0 = tx0 + (int64_t)(t % kLanesPerRow) * kRun;
+  const int32_t row = ty0 + t / kLanesPerRow;
+  const int32_t x0 = tx0 + (t % kLanesPerRow) * kRun;
   float2 acc[kRun];
 #pragma unroll
   for (int j = 0; j < kRun; ++j) acc[j] = make_float2(0.f, 0.f);
-  tile_sum(coef, b_begin, b_end, tx0, ty0, H, W, sh, acc);
+  tile_sum(coef, b_begin, b_end, tx0, ty0, (int32_t)H, (int32_t)W, sh, acc);
   if (row < H) {
     float2* dst = ws + ((int64_t)split * H + row) * W;
 #pragma unroll

@@ -230,27 +230,29 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
 
 // Pixels [xs, xe] of row `row` within the run [x0, x0 + kRun - 1] (and < W)
 // that lie inside the beamlet's cutoff ellipse; false when there are none.
-__device__ __forceinline__ bool row_extent(const tg_beamlet_coef& c, int64_t row, int64_t x0, int64_t W, int64_t& xs,
-                                           int64_t& xe) {
+// Pixel indices are 32-bit (images are far below 2^31 pixels per side).
+__device__ __forceinline__ bool row_extent(const tg_beamlet_coef& c, int32_t row, int32_t x0, int32_t W, int32_t& xs,
+                                           int32_t& xe) {
   xs = x0;
-  xe = x0 + kRun - 1;
-  if (xe >= W) xe = W - 1;
+  xe = min(x0 + kRun - 1, W - 1);
   if (c.gamma > 0.0f) {
     const float v = (float)row - c.cy;
     const float t2 = fmaf(-c.gamma * v, v, c.beta);
     if (t2 < 0.0f) return false;
-    const float sq = sqrtf(t2);
+    const float sq = __builtin_amdgcn_sqrtf(t2);
     const float mid = fmaf(-c.alpha, v, c.cx);
-    const float lo = ceilf(mid - sq), hi = floorf(mid + sq);
-    if (lo > (float)xs) xs = (lo > (float)(x0 + kRun)) ? x0 + kRun : (int64_t)lo;
-    if (hi < (float)xe) xe = (hi < (float)(x0 - 1)) ? x0 - 1 : (int64_t)hi;
+    // clamp in float first so the conversions stay in int32 range
+    const float lo = fminf(fmaxf(ceilf(mid - sq), (float)x0), (float)(x0 + kRun));
+    const float hi = fmaxf(fminf(floorf(mid + sq), (float)xe), (float)(x0 - 1));
+    xs = (int32_t)lo;
+    xe = (int32_t)hi;
   }
   return xs <= xe;
 }
 
 // exp(Phi) at (xs, row) and the first x-step ratio exp(Phi(xs + 1) - Phi(xs)),
 // with Phi evaluated in f64.
-__device__ __forceinline__ void run_start(const tg_beamlet_coef& c, int64_t xs, int64_t row, float2& F, float2& R) {
+__device__ __forceinline__ void run_start(const tg_beamlet_coef& c, int32_t xs, int32_t row, float2& F, float2& R) {
   const double px = (double)xs, py = (double)row;
   const double* a = c.phi;
   const double pre = ((a[0] * px + a[2] * py + a[6]) * px + (a[4] * py + a[8]) * py) + a[10];
@@ -273,11 +275,11 @@ struct TileLists {
 // cutoff box misses the tile are skipped after a deterministic ballot
 // compaction; every thread of the block must call this.
 __device__ __forceinline__ void tile_sum(const tg_beamlet_coef* __restrict__ coef, int64_t b_begin, int64_t b_end,
-                                         int64_t tx0, int64_t ty0, int64_t H, int64_t W, TileLists& sh,
+                                         int32_t tx0, int32_t ty0, int32_t H, int32_t W, TileLists& sh,
                                          float2 (&acc)[kRun]) {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int64_t row = ty0 + t / kLanesPerRow;
-  const int64_t x0 = tx0 + (int64_t)(t % kLanesPerRow) * kRun;
+  const int32_t row = ty0 + t / kLanesPerRow;
+  const int32_t x0 = tx0 + (t % kLanesPerRow) * kRun;
   const float ftx0 = (float)tx0, fty0 = (float)ty0, ftx1 = (float)(tx0 + kTileW - 1), fty1 = (float)(ty0 + kTileH - 1);
   for (int64_t b0 = b_begin; b0 < b_end; b0 += kThreads) {
     const int64_t b = b0 + t;
@@ -301,14 +303,14 @@ __device__ __forceinline__ void tile_sum(const tg_beamlet_coef* __restrict__ coe
       const int64_t bi = b0 + __builtin_amdgcn_readfirstlane(sh.list[q]);
       const tg_beamlet_coef& c = coef[bi];
       if (row >= H) continue;
-      int64_t xs, xe;
+      int32_t xs, xe;
       if (!row_extent(c, row, x0, W, xs, xe)) continue;
       float2 F, R;
       run_start(c, xs, row, F, R);
       const float2 K = make_float2(c.k_re, c.k_im);
 #pragma unroll
       for (int j = 0; j < kRun; ++j) {
-        const int64_t pxj = x0 + j;
+        const int32_t pxj = x0 + j;
         if (pxj >= xs && pxj <= xe) {
           acc[j].x += F.x;
           acc[j].y += F.y;

@@ -87,18 +87,18 @@ __global__ __launch_bounds__(kThreads) void fit_forward_kernel(const tg_beamlet_
   __shared__ TileLists sh;
   const FitItem it = items[blockIdx.x];
   const FitInst& I = insts[it.inst];
-  const int64_t H = I.H, W = I.W;
-  const int64_t tx0 = (int64_t)(it.tile % I.ntx) * kTileW, ty0 = (int64_t)(it.tile / I.ntx) * kTileH;
+  const int32_t H = I.H, W = I.W;
+  const int32_t tx0 = (it.tile % I.ntx) * kTileW, ty0 = (it.tile / I.ntx) * kTileH;
   const int64_t b_begin = I.g0 + I.n * it.slice / I.nslice, b_end = I.g0 + I.n * (it.slice + 1) / I.nslice;
   const int t = threadIdx.x;
-  const int64_t row = ty0 + t / kLanesPerRow;
-  const int64_t x0 = tx0 + (int64_t)(t % kLanesPerRow) * kRun;
+  const int32_t row = ty0 + t / kLanesPerRow;
+  const int32_t x0 = tx0 + (t % kLanesPerRow) * kRun;
   float2 acc[kRun];
 #pragma unroll
   for (int j = 0; j < kRun; ++j) acc[j] = make_float2(0.f, 0.f);
   tile_sum(coef, b_begin, b_end, tx0, ty0, H, W, sh, acc);
   if (row < H) {
-    float2* dst = partial + I.ws0 + ((int64_t)it.slice * H + row) * W;
+    float2* dst = partial + I.ws0 + ((int64_t)it.slice * H + row) * (int64_t)W;
 #pragma unroll
     for (int j = 0; j < kRun; ++j)
       if (x0 + j < W) dst[x0 + j] = acc[j];
@@ -197,22 +197,24 @@ __global__ __launch_bounds__(256) void fit_moment_kernel(const tg_beamlet_coef* 
     const float2 K = make_float2(c.k_re, c.k_im);
     const float2* wbase = weight + I.pix0;
     while (rr < nrows) {
-      const int64_t row = r0 + rr;
-      const int64_t x0 = c0 + (int64_t)run * kRun;
-      int64_t xs, xe;
+      const int32_t row = r0 + rr;
+      const int32_t x0 = c0 + run * kRun;
+      int32_t xs, xe;
       if (row_extent(c, row, x0, W, xs, xe)) {
-        // all 16 weights of the run at once (index clamped into the row),
-        // issued before the f64 run-start math so their latency overlaps it
-        const float2* wrow = wbase + row * (int64_t)W;
+        // all 16 weights of the run at once, issued before the f64 run-start
+        // math so their latency overlaps it; reads past the row end land in
+        // the next row or in the buffer's kRun-element tail padding and are
+        // never used (x <= xe <= W - 1)
+        const float2* wrun = wbase + (int64_t)row * W + x0;
         float2 w[kRun];
 #pragma unroll
-        for (int j = 0; j < kRun; ++j) w[j] = wrow[min(x0 + j, (int64_t)W - 1)];
+        for (int j = 0; j < kRun; ++j) w[j] = wrun[j];
         float2 F, R;
         run_start(c, xs, row, F, R);
         float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0;
 #pragma unroll
         for (int j = 0; j < kRun; ++j) {
-          const int64_t x = x0 + j;
+          const int32_t x = x0 + j;
           if (x >= xs && x <= xe) {
             const float2 P = cmulf(w[j], F);
             s0.x += P.x;
@@ -470,7 +472,7 @@ int build(const tg_fit_instance* inst, int32_t n_inst, int64_t n_gauss, double n
   p.off_coef = place((int64_t)sizeof(tg_beamlet_coef) * n_gauss);
   p.off_unit = place((int64_t)sizeof(tg_beamlet_coef) * n_gauss);
   p.off_partial = place((int64_t)sizeof(float2) * ws_elems);
-  p.off_weight = place((int64_t)sizeof(float2) * n_pix);
+  p.off_weight = place((int64_t)sizeof(float2) * (n_pix + kRun));  // + tail padding read by fit_moment_kernel
   p.off_loss_part = place((int64_t)sizeof(double) * n_blocks);
   p.off_moments = place((int64_t)sizeof(double) * kMoments * n_gauss);
   p.workspace_bytes = off;
@@ -508,6 +510,9 @@ extern "C" int tg_gauss_fit_upload(const tg_fit_plan* plan, const tg_fit_instanc
   put(plan->off_order, t.order.data(), t.order.size() * sizeof(int32_t));
   put(plan->off_gauss_inst, t.gauss_inst.data(), t.gauss_inst.size() * sizeof(int32_t));
   TG_HIP_CHECK(hipMemcpyAsync(workspace, host.data(), host.size(), hipMemcpyHostToDevice, (hipStream_t)stream));
+  // the weight buffer's tail padding is read (never used) by fit_moment_kernel
+  TG_HIP_CHECK(hipMemsetAsync(static_cast<char*>(workspace) + plan->off_weight + plan->n_pix * (int64_t)sizeof(float2), 0,
+                              kRun * sizeof(float2), (hipStream_t)stream));
   TG_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
   return TG_OK;
 }

@@ -39,6 +39,7 @@ def test_plan_counts_and_layout():
     assert offs == sorted(offs) and all(o % 256 == 0 for o in offs)
     assert plan.off_partial - plan.off_unit >= 136 * 2603 and plan.off_unit - plan.off_coef >= 136 * 2603
     assert plan.off_weight - plan.off_partial >= 8 * sum(s * n for s, n in zip(nslice, npix))
+    assert plan.off_loss_part - plan.off_weight >= 8 * (plan.n_pix + 16)  # tail padding of the weights
     assert plan.workspace_bytes >= plan.off_moments + 96 * 2603
     assert plan.table_bytes == plan.off_coef
 
