@@ -212,7 +212,7 @@ constexpr int kWaves = kThreads / 64;
 // in [-0.5, 0.5], accurate for |x| up to ~1e12 rad.
 __device__ __forceinline__ float revolutions(double x) {
   const double hi = x * kInv2PiHi;
-  const double err = fma(x, kInv2PiHi, -hi) + x * kInv2PiLo;
+  const double err = fma(x, kInv2PiLo, fma(x, kInv2PiHi, -hi));
   const double fr = hi - rint(hi);
   return (float)(fr + err);
 }
@@ -253,14 +253,14 @@ __device__ __forceinline__ bool row_extent(const tg_beamlet_coef& c, int32_t row
 // exp(Phi) at (xs, row) and the first x-step ratio exp(Phi(xs + 1) - Phi(xs)),
 // with Phi evaluated in f64.
 __device__ __forceinline__ void run_start(const tg_beamlet_coef& c, int32_t xs, int32_t row, float2& F, float2& R) {
-  const double px = (double)xs, py = (double)row;
+  const double px = (double)xs, py = (double)row, px2 = 2.0 * px + 1.0;
   const double* a = c.phi;
-  const double pre = ((a[0] * px + a[2] * py + a[6]) * px + (a[4] * py + a[8]) * py) + a[10];
-  const double pim = ((a[1] * px + a[3] * py + a[7]) * px + (a[5] * py + a[9]) * py) + a[11];
-  const double rre = a[0] * (2.0 * px + 1.0) + a[2] * py + a[6];
-  const double rim = a[1] * (2.0 * px + 1.0) + a[3] * py + a[7];
+  // Phi = (a px + b py + d) px + (c py + e) py + f, and Phi(px + 1) - Phi(px)
+  const double lre = fma(a[2], py, a[6]), lim = fma(a[3], py, a[7]);
+  const double pre = fma(fma(a[0], px, lre), px, fma(fma(a[4], py, a[8]), py, a[10]));
+  const double pim = fma(fma(a[1], px, lim), px, fma(fma(a[5], py, a[9]), py, a[11]));
   F = cexp_f(pre, pim);
-  R = cexp_f(rre, rim);
+  R = cexp_f(fma(a[0], px2, lre), fma(a[1], px2, lim));
 }
 
 // Shared memory of tile_sum.

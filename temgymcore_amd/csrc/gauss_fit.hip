@@ -201,48 +201,58 @@ __global__ __launch_bounds__(256) void fit_moment_kernel(const tg_beamlet_coef* 
       const int32_t x0 = c0 + run * kRun;
       int32_t xs, xe;
       if (row_extent(c, row, x0, W, xs, xe)) {
-        // all 16 weights of the run at once, issued before the f64 run-start
-        // math so their latency overlaps it; reads past the row end land in
-        // the next row or in the buffer's kRun-element tail padding and are
-        // never used (x <= xe <= W - 1)
-        const float2* wrun = wbase + (int64_t)row * W + x0;
+        // the pixels xs .. xe (at most kRun) of this run, as xs + k: their 16
+        // weights are loaded at once, before the f64 run-start math so that
+        // their latency overlaps it; reads past xe land in the row, the next
+        // row or the buffer's kRun-element tail padding (xs <= W - 1) and are
+        // masked out below
+        const float2* wrun = wbase + (int64_t)row * W + xs;
         float2 w[kRun];
 #pragma unroll
-        for (int j = 0; j < kRun; ++j) w[j] = wrun[j];
+        for (int k = 0; k < kRun; ++k) w[k] = wrun[k];
+        // keep all 16 loads ahead of the math: one memory round trip per run
+        __builtin_amdgcn_sched_barrier(0);
+        const int32_t cnt = xe - xs;  // last active k
         float2 F, R;
         run_start(c, xs, row, F, R);
         float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0;
+        // branch-free: steps past cnt are computed and discarded (select, so
+        // an overflowing tail cannot leak into the sums)
 #pragma unroll
-        for (int j = 0; j < kRun; ++j) {
-          const int32_t x = x0 + j;
-          if (x >= xs && x <= xe) {
-            const float2 P = cmulf(w[j], F);
-            s0.x += P.x;
-            s0.y += P.y;
-            s1.x = fmaf((float)j, P.x, s1.x);
-            s1.y = fmaf((float)j, P.y, s1.y);
-            s2.x = fmaf((float)(j * j), P.x, s2.x);
-            s2.y = fmaf((float)(j * j), P.y, s2.y);
+        for (int k = 0; k < kRun; ++k) {
+          float2 P = cmulf(w[k], F);
+          if (k > 0) {
+            const bool on = k <= cnt;
+            P.x = on ? P.x : 0.0f;
+            P.y = on ? P.y : 0.0f;
+          }
+          s0.x += P.x;
+          s0.y += P.y;
+          s1.x = fmaf((float)k, P.x, s1.x);
+          s1.y = fmaf((float)k, P.y, s1.y);
+          s2.x = fmaf((float)(k * k), P.x, s2.x);
+          s2.y = fmaf((float)(k * k), P.y, s2.y);
+          if (k + 1 < kRun) {
             F = cmulf(F, R);
             R = cmulf(R, K);
           }
         }
-        // run sums over j = x - x0 -> moments in u = x - pcx = ub + j, v = row - pcy
-        const double ub = (double)x0 - (double)pcx, v = (double)row - (double)pcy;
+        // run sums over k = x - xs -> moments in u = x - pcx = ub + k, v = row - pcy
+        const double ub = (double)xs - (double)pcx, v = (double)row - (double)pcy, vv = v * v;
         const double a0r = s0.x, a0i = s0.y, a1r = s1.x, a1i = s1.y, a2r = s2.x, a2i = s2.y;
-        const double u1r = ub * a0r + a1r, u1i = ub * a0i + a1i;  // sum u P
+        const double u1r = fma(ub, a0r, a1r), u1i = fma(ub, a0i, a1i);  // sum u P
         M[0] += a0r;
         M[1] += a0i;
         M[2] += u1r;
         M[3] += u1i;
-        M[4] += v * a0r;
-        M[5] += v * a0i;
-        M[6] += ub * (ub * a0r + 2.0 * a1r) + a2r;  // sum u^2 P
-        M[7] += ub * (ub * a0i + 2.0 * a1i) + a2i;
-        M[8] += v * u1r;
-        M[9] += v * u1i;
-        M[10] += v * v * a0r;
-        M[11] += v * v * a0i;
+        M[4] = fma(v, a0r, M[4]);
+        M[5] = fma(v, a0i, M[5]);
+        M[6] += fma(ub, u1r + a1r, a2r);  // sum u^2 P = ub (ub a0 + 2 a1) + a2
+        M[7] += fma(ub, u1i + a1i, a2i);
+        M[8] = fma(v, u1r, M[8]);
+        M[9] = fma(v, u1i, M[9]);
+        M[10] = fma(vv, a0r, M[10]);
+        M[11] = fma(vv, a0i, M[11]);
       }
       run += rem;
       rr += q;
