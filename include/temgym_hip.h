@@ -237,6 +237,25 @@ int64_t tg_beamlet_sum_workspace(int64_t n, int64_t height, int64_t width);
 int tg_beamlet_sum(const tg_beamlet_coef* coef, int64_t n, int64_t height, int64_t width, float* field,
                    void* workspace, int64_t workspace_bytes, void* stream);
 
+/* Gradient of a real loss L of that field with respect to the beamlet
+ * parameters (and, in propagated mode, the ray-transfer matrices): the
+ * backward of tg_beamlet_coefficients + tg_beamlet_sum, for make_gaussian_image
+ * / evaluate_gaussian_input_image under jax.grad (gaussian.py:225-427).
+ *   params, n, abcd, abcd_stride, mode, pix2m, nsigma: as for the forward
+ *   grad_field  [H, W] complex64: dL/dRe(field) + i dL/dIm(field)
+ *   grad_params [n, TG_BEAMLET_NPARAM] f64 (columns 13-15 are 0; the
+ *               amplitude as its real and imaginary parts)
+ *   grad_abcd   [n, 25] f64 per-beamlet dL/dT (row-major; the constant last
+ *               row is 0), propagated mode only, or NULL.  With a shared
+ *               matrix (abcd_stride 0) its gradient is the sum over beamlets.
+ * `workspace` needs tg_beamlet_backward_workspace(n, H, W) bytes, 256-byte
+ * aligned. */
+int64_t tg_beamlet_backward_workspace(int64_t n, int64_t height, int64_t width);
+int tg_beamlet_backward(const double* params, int64_t n, const double* abcd, int64_t abcd_stride, int32_t mode,
+                        const double* pix2m, double nsigma, int64_t height, int64_t width, const float* grad_field,
+                        double* grad_params, double* grad_abcd, void* workspace, int64_t workspace_bytes,
+                        void* stream);
+
 /* ======================================================================== *
  * Batched Gaussian-decomposition fits
  *

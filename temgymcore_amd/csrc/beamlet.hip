@@ -20,6 +20,18 @@
 // complex multiplicative recurrence F <- F R, R <- R K (K = exp(2a)): two
 // complex f32 products per pixel, no transcendental per pixel.  Partial tiles
 // of the beamlet slices are summed in a fixed order by a final kernel.
+//
+// Backward (tg_beamlet_backward): for a real loss L of the field, with the
+// cotangent g = dL/dRe(field) + i dL/dIm(field) and W = conj(g),
+//   dL/dtheta = Re sum_pixels W du_b/dtheta,  u_b = amp exp(Phi'_b(px, py)),
+// Phi' being Phi with unit amplitude.  One wavefront per beamlet reduces the
+// six complex moments sum W exp(Phi') {1, u, v, u^2, uv, v^2} over its cutoff
+// ellipse (the kernel shared with the batched fits); then, per beamlet in
+// f64, the moments give G_k = sum W exp(Phi') m_k for the six monomials
+// m_k = px^2, px py, py^2, px, py, 1 of Phi, and
+//   dL/dtheta = Re[amp sum_k G_k dc_k/dtheta]
+// with dc_k/dtheta from forward-mode dual numbers through the same
+// coefficient algebra (one pass per parameter and ray-transfer entry).
 #include "beamlet_common.hpp"
 
 namespace {
@@ -72,6 +84,117 @@ __global__ __launch_bounds__(256) void split_sum_kernel(const float2* __restrict
     im += v.y;
   }
   field[i] = make_float2((float)re, (float)im);
+}
+
+// ---------------------------------------------------------------------------
+// backward
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void beamlet_unit_kernel(const double* __restrict__ params, int64_t n,
+                                                           const double* __restrict__ abcd, int64_t abcd_stride,
+                                                           int32_t mode, double m0, double m1, double m2, double m3,
+                                                           double m4, double m5, float cut,
+                                                           tg_beamlet_coef* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* p = params + i * TG_BEAMLET_NPARAM;
+  const double m[6] = {m0, m1, m2, m3, m4, m5};
+  const double* T = mode == TG_BEAMLET_PROPAGATED ? abcd + i * abcd_stride : nullptr;
+  out[i] = make_coef(p, T, mode, m, mk(1.0, 0.0), cut);
+}
+
+// w = conj(g) over the image, then kRun zeros of tail padding
+__global__ __launch_bounds__(256) void conj_pad_kernel(const float2* __restrict__ g, int64_t npix,
+                                                       float2* __restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < npix) {
+    const float2 v = g[i];
+    w[i] = make_float2(v.x, -v.y);
+  } else if (i < npix + kRun) {
+    w[i] = make_float2(0.0f, 0.0f);
+  }
+}
+
+constexpr int kBeamletsPerMomentBlock = 4;  // one wavefront each
+
+__global__ __launch_bounds__(256) void beamlet_moment_kernel(const tg_beamlet_coef* __restrict__ unit, int64_t n,
+                                                             int32_t H, int32_t W, const float2* __restrict__ w,
+                                                             double* __restrict__ moments) {
+  const int lane = threadIdx.x & 63;
+  // one beamlet per wavefront: its index and record are wave-uniform
+  const int64_t b = (int64_t)blockIdx.x * kBeamletsPerMomentBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (b >= n) return;  // whole wavefronts leave; no block barrier below
+  double M[kMoments];
+  beamlet_moments(unit[b], H, W, w, lane, M);
+#pragma unroll
+  for (int k = 0; k < kMoments; ++k) M[k] = wave_sum_f64(M[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < kMoments; ++k) moments[b * kMoments + k] = M[k];
+  }
+}
+
+__global__ __launch_bounds__(64) void beamlet_grad_kernel(const double* __restrict__ params, int64_t n,
+                                                          const double* __restrict__ abcd, int64_t abcd_stride,
+                                                          int32_t mode, double m0, double m1, double m2, double m3,
+                                                          double m4, double m5, int32_t H, int32_t W,
+                                                          const tg_beamlet_coef* __restrict__ unit,
+                                                          const double* __restrict__ moments,
+                                                          double* __restrict__ grad_params,
+                                                          double* __restrict__ grad_abcd) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* p = params + i * TG_BEAMLET_NPARAM;
+  const bool prop = mode == TG_BEAMLET_PROPAGATED;
+  const double* T = prop ? abcd + i * abcd_stride : nullptr;
+  const double m[6] = {m0, m1, m2, m3, m4, m5};
+  float fcx, fcy;
+  centre_pixel(unit[i], H, W, fcx, fcy);
+  const double pcx = fcx, pcy = fcy;
+  const double* Mo = moments + i * kMoments;
+  const cd S0 = mk(Mo[0], Mo[1]), Su = mk(Mo[2], Mo[3]), Sv = mk(Mo[4], Mo[5]);
+  const cd Suu = mk(Mo[6], Mo[7]), Suv = mk(Mo[8], Mo[9]), Svv = mk(Mo[10], Mo[11]);
+  // G_k = sum W exp(Phi') m_k, monomials of (px, py) = (pcx + u, pcy + v)
+  cd G[6];
+  G[0] = add(add(Suu, scl(Su, 2.0 * pcx)), scl(S0, pcx * pcx));
+  G[1] = add(add(Suv, scl(Su, pcy)), add(scl(Sv, pcx), scl(S0, pcx * pcy)));
+  G[2] = add(add(Svv, scl(Sv, 2.0 * pcy)), scl(S0, pcy * pcy));
+  G[3] = add(Su, scl(S0, pcx));
+  G[4] = add(Sv, scl(S0, pcy));
+  G[5] = S0;
+  const double ar = p[5], ai = p[6];
+  double* gp = grad_params + i * TG_BEAMLET_NPARAM;
+  gp[5] = S0.re;   // u = amp exp(Phi'): dL/d amp_re = Re sum W exp(Phi')
+  gp[6] = -S0.im;  //                    dL/d amp_im = Re(i sum W exp(Phi'))
+  gp[13] = gp[14] = gp[15] = 0.0;
+  // directions: parameter columns 0-4 and 7-12, then ray-transfer entries 0-19
+  const int n_dir = prop ? 11 + 20 : 11;
+  for (int q = 0; q < n_dir; ++q) {
+    const int col = q < 5 ? q : (q < 11 ? q + 2 : -1);
+    const int ent = q < 11 ? -1 : q - 11;
+    dual pd[13], Td[25];
+#pragma unroll
+    for (int k = 0; k < 13; ++k) pd[k] = dual(p[k], k == col ? 1.0 : 0.0);
+    if (prop) {
+#pragma unroll
+      for (int e = 0; e < 25; ++e) Td[e] = dual(T[e], e == ent ? 1.0 : 0.0);
+    }
+    cx<dual> d6[6];
+    phi_coefficients<dual>(pd, Td, mode, m, mk(dual(1.0), dual(0.0)), d6);
+    double sr = 0.0, si = 0.0;  // sum_k G_k dc_k
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      sr += G[k].re * d6[k].re.t - G[k].im * d6[k].im.t;
+      si += G[k].re * d6[k].im.t + G[k].im * d6[k].re.t;
+    }
+    const double g = ar * sr - ai * si;  // Re[amp sum_k G_k dc_k]
+    if (col >= 0)
+      gp[col] = g;
+    else if (grad_abcd)
+      grad_abcd[i * 25 + ent] = g;
+  }
+  if (grad_abcd) {
+    for (int e = prop ? 20 : 0; e < 25; ++e) grad_abcd[i * 25 + e] = 0.0;
+  }
 }
 
 int splits_for(int64_t n, int64_t H, int64_t W) {
@@ -134,5 +257,58 @@ extern "C" int tg_beamlet_sum(const tg_beamlet_coef* coef, int64_t n, int64_t he
   hipLaunchKernelGGL(split_sum_kernel, dim3((unsigned)tg::cdiv(npix, 256)), dim3(256), 0, (hipStream_t)stream,
                      (const float2*)workspace, npix, (int32_t)nsplit, (float2*)field);
   TG_LAUNCH_CHECK("split_sum_kernel launch");
+  return TG_OK;
+}
+
+extern "C" int64_t tg_beamlet_backward_workspace(int64_t n, int64_t height, int64_t width) {
+  if (n < 0 || height <= 0 || width <= 0) return 0;
+  auto al = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
+  return al(n * (int64_t)sizeof(tg_beamlet_coef)) + al((height * width + kRun) * (int64_t)sizeof(float2)) +
+         al(n * kMoments * (int64_t)sizeof(double));
+}
+
+extern "C" int tg_beamlet_backward(const double* params, int64_t n, const double* abcd, int64_t abcd_stride,
+                                   int32_t mode, const double* pix2m, double nsigma, int64_t height, int64_t width,
+                                   const float* grad_field, double* grad_params, double* grad_abcd, void* workspace,
+                                   int64_t workspace_bytes, void* stream) {
+  tg::clear_error();
+  TG_REQUIRE(n >= 0 && height > 0 && width > 0, "tg_beamlet_backward: bad sizes");
+  TG_REQUIRE(height < ((int64_t)1 << 31) && width < ((int64_t)1 << 31) && height * width < ((int64_t)1 << 40),
+             "tg_beamlet_backward: image too large");
+  TG_REQUIRE(mode == TG_BEAMLET_PROPAGATED || mode == TG_BEAMLET_INPUT, "tg_beamlet_backward: bad mode");
+  TG_REQUIRE(pix2m != nullptr, "tg_beamlet_backward: NULL pixel transform");
+  TG_REQUIRE(grad_abcd == nullptr || mode == TG_BEAMLET_PROPAGATED,
+             "tg_beamlet_backward: ray-transfer gradients need propagated mode");
+  if (n == 0) return TG_OK;
+  TG_REQUIRE(params != nullptr && grad_field != nullptr && grad_params != nullptr && workspace != nullptr,
+             "tg_beamlet_backward: NULL buffer");
+  TG_REQUIRE(mode == TG_BEAMLET_INPUT || (abcd != nullptr && (abcd_stride == 0 || abcd_stride == 25)),
+             "tg_beamlet_backward: propagated mode needs abcd with stride 0 or 25");
+  TG_REQUIRE(reinterpret_cast<uintptr_t>(workspace) % 256 == 0, "tg_beamlet_backward: workspace not 256-byte aligned");
+  const int64_t need = tg_beamlet_backward_workspace(n, height, width);
+  TG_REQUIRE(workspace_bytes >= need, "tg_beamlet_backward: workspace %lld bytes < %lld", (long long)workspace_bytes,
+             (long long)need);
+  auto al = [](int64_t b) { return (b + 255) & ~(int64_t)255; };
+  char* ws = static_cast<char*>(workspace);
+  tg_beamlet_coef* unit = reinterpret_cast<tg_beamlet_coef*>(ws);
+  float2* w = reinterpret_cast<float2*>(ws + al(n * (int64_t)sizeof(tg_beamlet_coef)));
+  double* moments = reinterpret_cast<double*>(reinterpret_cast<char*>(w) +
+                                              al((height * width + kRun) * (int64_t)sizeof(float2)));
+  const float cut = nsigma > 0.0 ? (float)(nsigma * nsigma) : 80.0f;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(beamlet_unit_kernel, dim3((unsigned)tg::cdiv(n, 256)), dim3(256), 0, s, params, n, abcd,
+                     abcd_stride, mode, pix2m[0], pix2m[1], pix2m[2], pix2m[3], pix2m[4], pix2m[5], cut, unit);
+  TG_LAUNCH_CHECK("beamlet_unit_kernel launch");
+  const int64_t npix = height * width;
+  hipLaunchKernelGGL(conj_pad_kernel, dim3((unsigned)tg::cdiv(npix + kRun, 256)), dim3(256), 0, s,
+                     (const float2*)grad_field, npix, w);
+  TG_LAUNCH_CHECK("conj_pad_kernel launch");
+  hipLaunchKernelGGL(beamlet_moment_kernel, dim3((unsigned)tg::cdiv(n, kBeamletsPerMomentBlock)), dim3(256), 0, s,
+                     unit, n, (int32_t)height, (int32_t)width, w, moments);
+  TG_LAUNCH_CHECK("beamlet_moment_kernel launch");
+  hipLaunchKernelGGL(beamlet_grad_kernel, dim3((unsigned)tg::cdiv(n, 64)), dim3(64), 0, s, params, n, abcd,
+                     abcd_stride, mode, pix2m[0], pix2m[1], pix2m[2], pix2m[3], pix2m[4], pix2m[5], (int32_t)height,
+                     (int32_t)width, unit, moments, grad_params, grad_abcd);
+  TG_LAUNCH_CHECK("beamlet_grad_kernel launch");
   return TG_OK;
 }

@@ -43,7 +43,6 @@ constexpr int kMaxSlices = 8;           // Gaussian slices per instance tile
 constexpr int kSliceGauss = 256;        // target Gaussians per slice
 constexpr int kLossChunk = 4096;        // pixels per loss workgroup
 constexpr int kGaussPerMomentBlock = 4; // one wavefront each
-constexpr int kMoments = 12;            // 6 complex moments
 
 struct FitInst {
   int64_t g0, n, pix0, ws0;
@@ -105,12 +104,6 @@ __global__ __launch_bounds__(kThreads) void fit_forward_kernel(const tg_beamlet_
   }
 }
 
-__device__ __forceinline__ double wave_sum_f64(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
 __global__ __launch_bounds__(256) void fit_loss_kernel(const FitInst* __restrict__ insts,
                                                        const LossBlock* __restrict__ blocks,
                                                        const float2* __restrict__ partial,
@@ -156,14 +149,6 @@ __global__ __launch_bounds__(256) void fit_loss_final(const FitInst* __restrict_
   loss[i] = s / ((double)insts[i].H * (double)insts[i].W);
 }
 
-// Centre pixel of a Gaussian's moments: its envelope centre rounded and
-// clamped into the image (NaN -> 0).  Shared by the moment and gradient
-// kernels, which must agree on it.
-__device__ __forceinline__ void centre_pixel(const tg_beamlet_coef& c, int32_t H, int32_t W, float& pcx, float& pcy) {
-  pcx = fminf(fmaxf(rintf(c.cx), 0.0f), (float)(W - 1));
-  pcy = fminf(fmaxf(rintf(c.cy), 0.0f), (float)(H - 1));
-}
-
 __global__ __launch_bounds__(256) void fit_moment_kernel(const tg_beamlet_coef* __restrict__ unit,
                                                          const FitInst* __restrict__ insts,
                                                          const int32_t* __restrict__ gauss_inst,
@@ -175,93 +160,9 @@ __global__ __launch_bounds__(256) void fit_moment_kernel(const tg_beamlet_coef* 
   if (slot >= n_order) return;  // whole wavefronts leave; no block barrier below
   // one Gaussian per wavefront: its index and records are wave-uniform
   const int32_t g = __builtin_amdgcn_readfirstlane(order[slot]);
-  const tg_beamlet_coef& c = unit[g];
   const FitInst& I = insts[__builtin_amdgcn_readfirstlane(gauss_inst[g])];
-  const int32_t H = I.H, W = I.W;
-  // pixel box of the cutoff ellipse, clipped to the image (NaN -> whole image)
-  const float fc0 = fmaxf(ceilf(c.cx - c.box_w), 0.0f), fc1 = fminf(floorf(c.cx + c.box_w), (float)(W - 1));
-  const float fr0 = fmaxf(ceilf(c.cy - c.box_h), 0.0f), fr1 = fminf(floorf(c.cy + c.box_h), (float)(H - 1));
-  float pcx, pcy;
-  centre_pixel(c, H, W, pcx, pcy);
   double M[kMoments];
-#pragma unroll
-  for (int k = 0; k < kMoments; ++k) M[k] = 0.0;
-  if (fc0 <= fc1 && fr0 <= fr1) {
-    const int32_t c0 = (int32_t)fc0, r0 = (int32_t)fr0;
-    const int32_t nrows = (int32_t)fr1 - r0 + 1;
-    const int32_t rpr = ((int32_t)fc1 - c0 + kRun) / kRun;  // 16-pixel runs per row
-    // lane -> (row offset, run) of task `lane`, then advance by 64 tasks per
-    // step without dividing: 64 = q rpr + rem
-    int32_t rr = lane / rpr, run = lane - (lane / rpr) * rpr;
-    const int32_t q = 64 / rpr, rem = 64 - q * rpr;
-    const float2 K = make_float2(c.k_re, c.k_im);
-    const float2* wbase = weight + I.pix0;
-    while (rr < nrows) {
-      const int32_t row = r0 + rr;
-      const int32_t x0 = c0 + run * kRun;
-      int32_t xs, xe;
-      if (row_extent(c, row, x0, W, xs, xe)) {
-        // the pixels xs .. xe (at most kRun) of this run, as xs + k: their 16
-        // weights are loaded at once, before the f64 run-start math so that
-        // their latency overlaps it; reads past xe land in the row, the next
-        // row or the buffer's kRun-element tail padding (xs <= W - 1) and are
-        // masked out below
-        const float2* wrun = wbase + (int64_t)row * W + xs;
-        float2 w[kRun];
-#pragma unroll
-        for (int k = 0; k < kRun; ++k) w[k] = wrun[k];
-        // keep all 16 loads ahead of the math: one memory round trip per run
-        __builtin_amdgcn_sched_barrier(0);
-        const int32_t cnt = xe - xs;  // last active k
-        float2 F, R;
-        run_start(c, xs, row, F, R);
-        float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0;
-        // branch-free: steps past cnt are computed and discarded (select, so
-        // an overflowing tail cannot leak into the sums)
-#pragma unroll
-        for (int k = 0; k < kRun; ++k) {
-          float2 P = cmulf(w[k], F);
-          if (k > 0) {
-            const bool on = k <= cnt;
-            P.x = on ? P.x : 0.0f;
-            P.y = on ? P.y : 0.0f;
-          }
-          s0.x += P.x;
-          s0.y += P.y;
-          s1.x = fmaf((float)k, P.x, s1.x);
-          s1.y = fmaf((float)k, P.y, s1.y);
-          s2.x = fmaf((float)(k * k), P.x, s2.x);
-          s2.y = fmaf((float)(k * k), P.y, s2.y);
-          if (k + 1 < kRun) {
-            F = cmulf(F, R);
-            R = cmulf(R, K);
-          }
-        }
-        // run sums over k = x - xs -> moments in u = x - pcx = ub + k, v = row - pcy
-        const double ub = (double)xs - (double)pcx, v = (double)row - (double)pcy, vv = v * v;
-        const double a0r = s0.x, a0i = s0.y, a1r = s1.x, a1i = s1.y, a2r = s2.x, a2i = s2.y;
-        const double u1r = fma(ub, a0r, a1r), u1i = fma(ub, a0i, a1i);  // sum u P
-        M[0] += a0r;
-        M[1] += a0i;
-        M[2] += u1r;
-        M[3] += u1i;
-        M[4] = fma(v, a0r, M[4]);
-        M[5] = fma(v, a0i, M[5]);
-        M[6] += fma(ub, u1r + a1r, a2r);  // sum u^2 P = ub (ub a0 + 2 a1) + a2
-        M[7] += fma(ub, u1i + a1i, a2i);
-        M[8] = fma(v, u1r, M[8]);
-        M[9] = fma(v, u1i, M[9]);
-        M[10] = fma(vv, a0r, M[10]);
-        M[11] = fma(vv, a0i, M[11]);
-      }
-      run += rem;
-      rr += q;
-      if (run >= rpr) {
-        run -= rpr;
-        ++rr;
-      }
-    }
-  }
+  beamlet_moments(unit[g], I.H, I.W, weight + I.pix0, lane, M);
 #pragma unroll
   for (int k = 0; k < kMoments; ++k) M[k] = wave_sum_f64(M[k]);
   if (lane == 0) {

@@ -12,6 +12,12 @@ The per-beamlet 5x5 ray-transfer matrix comes from ``transfer.jacobian`` of
 ``run_to_end`` over the model (reference gaussian.py:234-238 takes
 ``jax.jacobian``), evaluated once per distinct input-plane z -- exact for
 models built from the affine elements (every element except Biprism).
+
+Both images are differentiable: when a GaussianRay field (a torch tensor with
+requires_grad) or an explicit ``abcd`` tensor requires a gradient, the field
+is produced by an autograd function whose backward is the hand-written
+adjoint ``tg_beamlet_backward`` (the reference differentiates these images
+with jax.grad).
 """
 from __future__ import annotations
 
@@ -45,6 +51,9 @@ _lib.register_signatures({
     "tg_beamlet_sum_workspace": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
     "tg_beamlet_sum": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _P, _P, ctypes.c_int64,
                                       _P]),
+    "tg_beamlet_backward_workspace": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
+    "tg_beamlet_backward": (ctypes.c_int, [_P, ctypes.c_int64, _P, ctypes.c_int64, ctypes.c_int32, _P, ctypes.c_double,
+                                           ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P, ctypes.c_int64, _P]),
 })
 
 MODE_PROPAGATED = 0
@@ -228,8 +237,89 @@ class BeamletSum:
         return torch.view_as_complex(self.field)
 
 
-def make_gaussian_image(gaussian_rays: GaussianRay, model, batch_size=128, cutoff: float | None = None):
-    """Complex field of all beamlets after `model` on its last element (a Grid)."""
+def beamlet_field(params: torch.Tensor, grid, mode: int, abcd: torch.Tensor | None = None, abcd_stride: int = 0,
+                  cutoff: float | None = None) -> torch.Tensor:
+    """complex64 field [H, W] of the beamlets params [n, 16] on `grid`."""
+    params = params.detach().contiguous()
+    abcd = None if abcd is None else abcd.detach().contiguous()
+    coef = beamlet_coefficients(params, grid, mode, abcd, abcd_stride, cutoff)
+    return BeamletSum(params.shape[0], grid.shape, params.device).run(coef).clone()
+
+
+def beamlet_backward(params: torch.Tensor, grid, mode: int, grad_field: torch.Tensor,
+                     abcd: torch.Tensor | None = None, abcd_stride: int = 0, cutoff: float | None = None,
+                     want_abcd: bool = False):
+    """Adjoint of ``beamlet_field``: (dL/dparams [n, 16] f64, dL/dT [n, 25] f64
+    or None) for the field cotangent grad_field = dL/dRe + i dL/dIm."""
+    lib = _lib.load()
+    params = params.detach().contiguous()
+    n = int(params.shape[0])
+    H, W = int(grid.shape[0]), int(grid.shape[1])
+    g = grad_field.detach().to(device=params.device, dtype=torch.complex64).resolve_conj().contiguous()
+    if tuple(g.shape) != (H, W):
+        raise ValueError(f"beamlet_backward: grad_field shape {tuple(g.shape)} != grid shape {(H, W)}")
+    grad = torch.empty((n, NPARAM), dtype=torch.float64, device=params.device)
+    gabcd = None
+    if want_abcd and mode == MODE_PROPAGATED:
+        gabcd = torch.empty((n, 25), dtype=torch.float64, device=params.device)
+    ws_bytes = int(lib.tg_beamlet_backward_workspace(n, H, W))
+    raw = torch.empty(ws_bytes + 256, dtype=torch.uint8, device=params.device)
+    off = (-raw.data_ptr()) % 256
+    m = _pix2m(grid)
+    abcd = None if abcd is None else abcd.detach().contiguous()
+    st = lib.tg_beamlet_backward(params.data_ptr(), n, abcd.data_ptr() if abcd is not None else None, abcd_stride,
+                                 mode, m.ctypes.data, float(cutoff or 0.0), H, W,
+                                 torch.view_as_real(g).data_ptr(), grad.data_ptr(),
+                                 gabcd.data_ptr() if gabcd is not None else None, raw.data_ptr() + off, ws_bytes,
+                                 _lib.stream_ptr(params.device))
+    _lib.check(st, "tg_beamlet_backward")
+    return grad, gabcd
+
+
+class _BeamletField(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, params, abcd, grid, mode, abcd_stride, cutoff):
+        ctx.save_for_backward(params, abcd)
+        ctx.grid, ctx.mode, ctx.abcd_stride, ctx.cutoff = grid, mode, abcd_stride, cutoff
+        return beamlet_field(params, grid, mode, abcd, abcd_stride, cutoff)
+
+    @staticmethod
+    def backward(ctx, grad_field):
+        params, abcd = ctx.saved_tensors
+        want_abcd = abcd is not None and ctx.needs_input_grad[1]
+        gp, ga = beamlet_backward(params, ctx.grid, ctx.mode, grad_field, abcd, ctx.abcd_stride, ctx.cutoff,
+                                  want_abcd=want_abcd)
+        g_abcd = None
+        if want_abcd:
+            ga = ga.view(-1, 5, 5)
+            g_abcd = ga.sum(0, keepdim=True) if ctx.abcd_stride == 0 else ga
+            g_abcd = g_abcd.reshape(abcd.shape).to(abcd.dtype)
+        return gp.to(params.dtype), g_abcd, None, None, None, None
+
+
+def _field(params, grid, mode, abcd, stride, cutoff):
+    needs = torch.is_grad_enabled() and (params.requires_grad or (abcd is not None and abcd.requires_grad))
+    if needs:
+        return _BeamletField.apply(params, abcd, grid, mode, stride, cutoff)
+    return beamlet_field(params, grid, mode, abcd, stride, cutoff)
+
+
+def _user_abcd(abcd, n, device):
+    a = abcd if isinstance(abcd, torch.Tensor) else torch.as_tensor(np.asarray(abcd, dtype=np.float64))
+    a = a.to(device=device, dtype=torch.float64)
+    if a.shape == (5, 5) or a.shape == (1, 5, 5):
+        return a.reshape(1, 5, 5), 0
+    if a.shape == (n, 5, 5):
+        return a, 25
+    raise ValueError(f"abcd: expected [5, 5] or [n, 5, 5], got {tuple(a.shape)}")
+
+
+def make_gaussian_image(gaussian_rays: GaussianRay, model, batch_size=128, cutoff: float | None = None, *,
+                        abcd=None):
+    """Complex field of all beamlets after `model` on its last element (a Grid).
+
+    ``abcd`` ([5, 5] or [n, 5, 5]) replaces the model's ray-transfer matrix,
+    e.g. a tensor built from differentiable lens parameters."""
     from .grid import Grid
 
     rays = gaussian_rays
@@ -238,9 +328,12 @@ def make_gaussian_image(gaussian_rays: GaussianRay, model, batch_size=128, cutof
     assert isinstance(grid, Grid)
     device = _lib.require_device()
     params = pack_params(rays.to_vector(), device)
-    abcd, stride = _abcd_per_beamlet(rays, model, device, int(params.shape[0]))
-    coef = beamlet_coefficients(params, grid, MODE_PROPAGATED, abcd, stride, cutoff)
-    return BeamletSum(params.shape[0], grid.shape, device).run(coef).clone()
+    n = int(params.shape[0])
+    if abcd is None:
+        abcd, stride = _abcd_per_beamlet(rays, model, device, n)
+    else:
+        abcd, stride = _user_abcd(abcd, n, device)
+    return _field(params, grid, MODE_PROPAGATED, abcd, stride, cutoff)
 
 
 def evaluate_gaussian_input_image(gaussian_rays: GaussianRay, grid, batch_size=128, cutoff: float | None = None):
@@ -249,5 +342,4 @@ def evaluate_gaussian_input_image(gaussian_rays: GaussianRay, grid, batch_size=1
     assert isinstance(rays, GaussianRay)
     device = _lib.require_device()
     params = pack_params(rays.to_vector(), device)
-    coef = beamlet_coefficients(params, grid, MODE_INPUT, None, 0, cutoff)
-    return BeamletSum(params.shape[0], grid.shape, device).run(coef).clone()
+    return _field(params, grid, MODE_INPUT, None, 0, cutoff)
