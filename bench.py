@@ -41,6 +41,20 @@ HBM_PEAK = 8.0e12  # B/s, MI355X
 SKIP_CPU = False
 
 
+def _profiled_traffic(fname):
+    """HBM bytes per launch of a kernel as measured by the calibrated
+    FETCH_SIZE + WRITE_SIZE PMC runs summarised in profiles/<fname>
+    (rocprofv3 cannot run inside this process); None if absent."""
+    try:
+        with open(os.path.join(ROOT, "profiles", fname)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def _events(n):
     import torch
 
@@ -95,7 +109,8 @@ def bench_plain(steps: int, warmup: int):
         config={"workload": "plain run_to_image", "rays": n, "detector": [128, 128], "seed": 0,
                 "kernel_ms": kern_ms, "parity_x_bit_exact": parity},
         roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK, "traffic": None},
+                  "frac": achieved / HBM_PEAK, "traffic": _profiled_traffic("ray_program_kernel-plain_1M.md"),
+                  "traffic_source": "profiles/ray_program_kernel-plain_1M.md (calibrated FETCH_SIZE + WRITE_SIZE)"},
         cpu_baseline=None if SKIP_CPU else {"value": n / cpu_s, "unit": "rays/s", "cores": 1, "kind": "port",
                                             "sample": f"numpy oracle, full {n}-ray batch x {reps} reps"},
     )
@@ -248,7 +263,9 @@ def _result_fit(units, ms_per_step, kern_ms, graph_ms, gpu_out, achieved, probe_
                 "kernels_per_step": "stem_fit_kernel + row_sum_kernel (event time covers both)",
                 "cpu_port_rel_diff_on_sample": rel, "loss": float(gpu_out[-1])},
         roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK, "traffic": None,
+                  "frac": achieved / HBM_PEAK,
+                  "traffic": _profiled_traffic("stem_fit_kernel-stem_fit_64x64x4096x8.md"),
+                  "traffic_source": "profiles/stem_fit_kernel-stem_fit_64x64x4096x8.md (calibrated FETCH_SIZE + WRITE_SIZE)",
                   "measured_read_peak": probe_bw / 1e9 if probe_ok else None,
                   "frac_of_measured": achieved / probe_bw if probe_ok else None},
         cpu_baseline=cpu,
