@@ -76,26 +76,52 @@ def bench_plain(steps: int, warmup: int):
     t = [torch.from_numpy(a).to(dev) for a in (x, y, dx, dy)]
     model = [Lens(z=0.5, focal_length=1.0), Detector(z=1.0, pixel_size=(0.01, 0.01), shape=(128, 128))]
     plan = plan_run_to_image(Ray(x=t[0], y=t[1], dx=t[2], dy=t[3], z=0.0, pathlength=0.0), model)
+    from temgymcore_amd import autotune
+
+    tuned_grid, tune_ms = autotune.tune_run_to_image(plan)
     for _ in range(warmup):
         plan.acc.zero_()
         plan.launch()
     torch.cuda.synchronize()
+    # kernel time: events around the launch alone
     ev = _events(2 * steps)
-    t0 = time.perf_counter()
     for i in range(steps):
         plan.acc.zero_()
         ev[2 * i].record()
         plan.launch()
         ev[2 * i + 1].record()
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     kern_ms = sum(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(steps)) / steps
+
+    # the timed steps: one step (zero the histogram + the fused launch) is
+    # captured once in a HIP graph and replayed K times
+    def step():
+        plan.acc.zero_()
+        plan.launch()
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step()
+    for _ in range(max(warmup, 1)):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / steps
     bytes_per_launch = 36 * n + 128 * 128 * 4  # 16 B in + 20 B out per ray, + histogram
     achieved = bytes_per_launch / (kern_ms * 1e-3)
     # parity spot check of the timed buffers
     ref, hist = O.plain_run(x, y, dx, dy)
-    parity = bool(np.array_equal(plan.out_ray.x.cpu().numpy(), ref["x"]))
+    parity = bool(np.array_equal(plan.out_ray.x.cpu().numpy(), ref["x"]) and
+                  np.array_equal(plan.acc.cpu().numpy().astype(np.int64), np.asarray(hist, dtype=np.int64)))
 
     # CPU baseline: the numpy oracle (single thread) on the full batch, ~10 s
     reps, c0 = 0, time.perf_counter()
@@ -107,7 +133,9 @@ def bench_plain(steps: int, warmup: int):
         metric="rays/s (plain run_to_image: 1,048,576 f32 rays, Lens + Detector 128x128 binning)",
         value=n / (ms_per_step * 1e-3), unit="rays/s", ms_per_step=ms_per_step, dtype="f32",
         config={"workload": "plain run_to_image", "rays": n, "detector": [128, 128], "seed": 0,
-                "kernel_ms": kern_ms, "parity_x_bit_exact": parity},
+                "kernel_ms": kern_ms, "parity_x_and_histogram_bit_exact": parity,
+                "timed_as": "K HIP-graph replays of (histogram zero + fused launch)",
+                "autotune": {"ray_grid": tuned_grid, "candidate_ms": tune_ms}},
         roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                   "frac": achieved / HBM_PEAK, "traffic": _profiled_traffic("ray_program_kernel-plain_1M.md"),
                   "traffic_source": "profiles/ray_program_kernel-plain_1M.md (calibrated FETCH_SIZE + WRITE_SIZE)"},
@@ -366,6 +394,11 @@ def bench_beamlet(steps: int, warmup: int):
     det = Detector(z=L, pixel_size=(pix, pix), shape=(H, W))
     params = pack_params(rays, dev)
     abcd, stride = _abcd_per_beamlet(rays, [det], dev, n)
+    from temgymcore_amd import autotune
+
+    coef0 = beamlet_coefficients(params, det, MODE_PROPAGATED, abcd, stride, cutoff=4.0)
+    tuned_splits, tune_ms = autotune.tune_beamlet_sum(coef0, n, (H, W))
+    del coef0
     summer = BeamletSum(n, (H, W), dev)
     holder = {}
 
@@ -403,7 +436,8 @@ def bench_beamlet(steps: int, warmup: int):
         config={"workload": "beamlet: 131072 beamlets, 10 um aperture, w0 50 nm, 0.3 m, 1024^2 px of 40 nm",
                 "evaluated_pairs": evaluated, "dense_pairs": dense, "dense_equiv_evals_per_s": dense / (ms_per_step * 1e-3),
                 "event_ms_per_step": kern_ms, "graph_replay_ms_per_step": graph_ms,
-                "kernels_per_step": "beamlet_coef_kernel + beamlet_sum_kernel + split_sum_kernel"},
+                "kernels_per_step": "beamlet_coef_kernel + beamlet_sum_kernel + split_sum_kernel",
+                "autotune": {"beamlet_splits": tuned_splits, "candidate_ms": tune_ms}},
         roofline={"bound": "valu", "achieved": achieved / 1e12, "peak": 157.3, "unit": "TFLOP/s",
                   "frac": achieved / 157.3e12, "traffic": None,
                   "flops_per_eval": flops_per_eval},
@@ -419,7 +453,10 @@ def bench_gauss_fit(steps: int, warmup: int):
     from oracle import gauss_fit_oracle as GF
     from temgymcore_amd import gauss_fit
 
+    from temgymcore_amd import autotune
+
     batch, params, target = gauss_fit.workload()
+    tuned_slice, tune_ms, batch = autotune.tune_gauss_fit(batch.grids, batch.counts, params, target)
     dev = batch.device
     loss = torch.empty(batch.n_inst, dtype=torch.float64, device=dev)
     grad = torch.empty((batch.n_gauss, 16), dtype=torch.float64, device=dev)
@@ -478,7 +515,8 @@ def bench_gauss_fit(steps: int, warmup: int):
                 "gaussians": batch.n_gauss, "pixels": batch.n_pix, "dense_pairs": dense,
                 "evaluated_pairs_per_pass": evaluated, "event_ms_per_step": kern_ms,
                 "graph_replay_ms_per_step": graph_ms, "forward_only_event_ms": fwd_kern_ms,
-                "kernels_per_step": "fit_coef, fit_forward, fit_loss, fit_loss_final, fit_moment, fit_grad"},
+                "kernels_per_step": "fit_coef, fit_forward, fit_loss, fit_loss_final, fit_moment, fit_grad",
+                "autotune": {"fit_slice": tuned_slice, "candidate_ms": tune_ms}},
         roofline={"bound": "valu", "achieved": achieved / 1e12, "peak": 157.3, "unit": "TFLOP/s",
                   "frac": achieved / 157.3e12, "traffic": None,
                   "flops_per_eval": {"forward": flops_fwd, "backward": flops_bwd},

@@ -129,6 +129,24 @@ int tg_ray_program_run(const tg_program* prog, const tg_bin_spec* bin, void* str
 /* Number of compute units of the current device (cached per device). */
 int32_t tg_device_cu_count(void);
 
+/* ---- autotuning ----------------------------------------------------------
+ * A per-process table of launch choices, keyed by (knob, current device,
+ * key[4]).  The autotuner (temgymcore_amd/autotune.py) times candidates and
+ * records the fastest; the entry points below consult the table and fall back
+ * to built-in heuristics.  Thread-safe.  No reference counterpart (the
+ * reference leaves launch shapes to XLA).
+ *   TG_TUNE_BEAMLET_SPLITS key {n, H, W, 0}: beamlet slices of tg_beamlet_sum
+ *                          (and of tg_beamlet_sum_workspace), 1..128
+ *   TG_TUNE_RAY_GRID       key {n, H*W, dtype, 0}: workgroups of a binning
+ *                          tg_ray_program_run whose bin->grid_hint is 0
+ *   TG_TUNE_FIT_SLICE      key {n_inst, n_gauss, n_pix, 0}: target Gaussians
+ *                          per forward slice in tg_gauss_fit_plan, 16..4096 */
+enum tg_tune_knob { TG_TUNE_BEAMLET_SPLITS = 1, TG_TUNE_RAY_GRID = 2, TG_TUNE_FIT_SLICE = 3 };
+int tg_tune_set(int32_t knob, const int64_t* key, int32_t value);
+/* 1 and *value set when an entry exists, 0 when not, < 0 on error. */
+int tg_tune_get(int32_t knob, const int64_t* key, int32_t* value);
+int tg_tune_clear(void);
+
 /* ======================================================================== *
  * 4D-STEM shadow-image model (scan x camera-length batches)
  *

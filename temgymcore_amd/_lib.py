@@ -83,6 +83,9 @@ _SIGNATURES: dict[str, tuple] = {
     "tg_last_error": (ctypes.c_char_p, []),
     "tg_abi_version": (ctypes.c_int32, []),
     "tg_device_cu_count": (ctypes.c_int32, []),
+    "tg_tune_set": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
+    "tg_tune_get": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
+    "tg_tune_clear": (ctypes.c_int, []),
     "tg_ray_program_run": (ctypes.c_int, [ctypes.POINTER(Program), ctypes.POINTER(BinSpec), ctypes.c_void_p]),
     "tg_hbm_read_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32,
                                          ctypes.c_void_p]),

@@ -1,0 +1,131 @@
+"""Autotuner of launch shapes.
+
+Each tuner times a few candidate launch choices of one kernel on the real
+inputs (HIP events on torch's current stream, median of `reps` runs after a
+warm-up) and records the fastest in the library's tuning table
+(``tg_tune_set``), which the entry points consult on every later call with
+the same problem sizes.  Tuning is explicit: nothing here runs unless called,
+and ``clear()`` restores the built-in heuristics.
+
+Knobs (include/temgym_hip.h, ``tg_tune_knob``):
+  BEAMLET_SPLITS  beamlet slices per detector tile of ``tg_beamlet_sum``
+  RAY_GRID        workgroups of a binning ``tg_ray_program_run``
+  FIT_SLICE       Gaussians per forward slice of the batched fits
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+
+import torch
+
+from . import _lib
+
+BEAMLET_SPLITS, RAY_GRID, FIT_SLICE = 1, 2, 3
+_NAMES = {BEAMLET_SPLITS: "beamlet_splits", RAY_GRID: "ray_grid", FIT_SLICE: "fit_slice"}
+
+
+def _key(key):
+    k = [int(v) for v in key] + [0] * (4 - len(key))
+    return (ctypes.c_int64 * 4)(*k[:4])
+
+
+def set_choice(knob: int, key, value: int) -> None:
+    _lib.check(_lib.load().tg_tune_set(knob, _key(key), int(value)), "tg_tune_set")
+
+
+def get_choice(knob: int, key) -> int | None:
+    v = ctypes.c_int32(0)
+    st = _lib.load().tg_tune_get(knob, _key(key), ctypes.byref(v))
+    if st < 0:
+        _lib.check(st, "tg_tune_get")
+    return int(v.value) if st == 1 else None
+
+
+def clear() -> None:
+    _lib.load().tg_tune_clear()
+
+
+def time_ms(fn, reps: int = 5, warmup: int = 1) -> float:
+    """Median GPU time of fn() in ms (HIP events on the current stream)."""
+    for _ in range(warmup):
+        fn()
+    times = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    return sorted(times)[len(times) // 2]
+
+
+def _pick(knob, key, candidates, run, reps):
+    """Time run() under each candidate choice; keep and return the fastest."""
+    results = {}
+    for c in candidates:
+        set_choice(knob, key, c)
+        results[int(c)] = time_ms(run, reps)
+    best = min(results, key=results.get)
+    set_choice(knob, key, best)
+    return best, results
+
+
+def tune_beamlet_sum(coef: torch.Tensor, n: int, shape, candidates=(8, 16, 32, 64, 128), reps: int = 3):
+    """Beamlet slices for tg_beamlet_sum(n, H, W) on these coefficient records."""
+    from .gaussian import BeamletSum
+
+    summer = BeamletSum(n, shape, coef.device)
+    cands = [c for c in candidates if c <= max(n, 1)] or [1]
+    return _pick(BEAMLET_SPLITS, (n, shape[0], shape[1]), cands, lambda: summer.run(coef), reps)
+
+
+def tune_run_to_image(plan, candidates=None, reps: int = 5):
+    """Workgroups of a fused run_to_image plan (binning programs only)."""
+    binning = [(prog, bs) for prog, bs, _ in plan.programs if bs is not None and bs.enabled]
+    if not binning:
+        raise ValueError("tune_run_to_image: the plan does not bin")
+    prog, bs = binning[0]
+    cus = int(_lib.load().tg_device_cu_count())
+    if candidates is None:
+        candidates = [cus // 2, cus, 2 * cus, 4 * cus]
+    key = (prog.n, bs.height * bs.width, prog.dtype)
+    return _pick(RAY_GRID, key, [c for c in candidates if c > 0], lambda: plan.launch(), reps)
+
+
+def tune_gauss_fit(grids, counts, params: torch.Tensor, target: torch.Tensor, candidates=(128, 256, 512, 1024),
+                   cutoff=None, reps: int = 3):
+    """Gaussians per forward slice for this batch geometry; returns
+    (best, timings, batch planned with the best choice)."""
+    from .gauss_fit import GaussianFitBatch
+
+    npix = sum(int(g.shape[0]) * int(g.shape[1]) for g in grids)
+    key = (len(grids), int(sum(counts)), npix)
+    loss = torch.empty(len(grids), dtype=torch.float64, device=params.device)
+    grad = torch.empty((int(sum(counts)), 16), dtype=torch.float64, device=params.device)
+    results = {}
+    for c in candidates:
+        set_choice(FIT_SLICE, key, c)
+        b = GaussianFitBatch(grids, counts, cutoff)
+        results[int(c)] = time_ms(lambda: b.run(params, target, None, loss, grad), reps)
+        del b
+    best = min(results, key=results.get)
+    set_choice(FIT_SLICE, key, best)
+    return best, results, GaussianFitBatch(grids, counts, cutoff)
+
+
+def save(path: str, entries) -> None:
+    """Write [(knob, key, value), ...] as JSON (for reuse across processes)."""
+    with open(path, "w") as f:
+        json.dump([{"knob": _NAMES[k], "key": list(map(int, key)), "value": int(v)} for k, key, v in entries], f)
+
+
+def load(path: str) -> int:
+    """Record every entry of a file written by save(); returns the count."""
+    inv = {v: k for k, v in _NAMES.items()}
+    with open(path) as f:
+        rows = json.load(f)
+    for r in rows:
+        set_choice(inv[r["knob"]], r["key"], r["value"])
+    return len(rows)

@@ -198,6 +198,15 @@ __global__ __launch_bounds__(64) void beamlet_grad_kernel(const double* __restri
 }
 
 int splits_for(int64_t n, int64_t H, int64_t W) {
+  const int64_t key[4] = {n, H, W, 0};
+  int32_t tuned = 0;
+  if (tg::tune_lookup(TG_TUNE_BEAMLET_SPLITS, key, tuned)) {
+    int64_t s = tuned;
+    if (s > 128) s = 128;
+    if (s > n && n > 0) s = n;
+    if (s < 1) s = 1;
+    return (int)s;
+  }
   const int64_t ntiles = tg::cdiv(H, kTileH) * tg::cdiv(W, kTileW);
   // Many small work items (tile x beamlet slice) let the hardware scheduler
   // balance tiles near the centre of the pattern, which overlap many more

@@ -264,8 +264,8 @@ struct Tables {
   std::vector<int32_t> gauss_inst;
 };
 
-int slices_for(int64_t n) {
-  int64_t s = tg::cdiv(n, kSliceGauss);
+int slices_for(int64_t n, int64_t slice_gauss) {
+  int64_t s = tg::cdiv(n, slice_gauss);
   if (s > kMaxSlices) s = kMaxSlices;
   if (s < 1) s = 1;
   return (int)s;
@@ -305,6 +305,12 @@ int build(const tg_fit_instance* inst, int32_t n_inst, int64_t n_gauss, double n
                    inst[idx[k]].n_gauss == 0,
                "tg_gauss_fit_plan: instances %d and %d share Gaussian rows", idx[k - 1], idx[k]);
   }
+  int64_t slice_gauss = kSliceGauss;
+  {
+    const int64_t key[4] = {n_inst, n_gauss, n_pix, 0};
+    int32_t tuned = 0;
+    if (tg::tune_lookup(TG_TUNE_FIT_SLICE, key, tuned)) slice_gauss = std::min<int64_t>(std::max(tuned, 16), 4096);
+  }
   Tables t;
   t.inst.resize(n_inst);
   t.gauss_inst.assign(n_gauss, -1);
@@ -320,7 +326,7 @@ int build(const tg_fit_instance* inst, int32_t n_inst, int64_t n_gauss, double n
     f.H = s.height;
     f.W = s.width;
     f.ntx = (int32_t)tg::cdiv(s.width, kTileW);
-    f.nslice = slices_for(s.n_gauss);
+    f.nslice = slices_for(s.n_gauss, slice_gauss);
     const int64_t npix = (int64_t)s.height * s.width;
     ws_elems += (int64_t)f.nslice * npix;
     n_items += tg::cdiv(s.height, kTileH) * f.ntx * f.nslice;
@@ -410,7 +416,7 @@ extern "C" int tg_gauss_fit_upload(const tg_fit_plan* plan, const tg_fit_instanc
   if (rc != TG_OK) return rc;
   TG_REQUIRE(check.table_bytes == plan->table_bytes && check.workspace_bytes == plan->workspace_bytes &&
                  check.n_items == plan->n_items && check.n_pix == plan->n_pix,
-             "tg_gauss_fit_upload: instances differ from the plan's");
+             "tg_gauss_fit_upload: instances (or the tuned slicing) differ from the plan's");
   std::vector<char> host((size_t)plan->table_bytes, 0);
   auto put = [&](int64_t off, const void* src, size_t bytes) {
     if (bytes) memcpy(host.data() + off, src, bytes);

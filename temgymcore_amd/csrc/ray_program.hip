@@ -304,7 +304,13 @@ int launch(const tg_program& P, const tg_bin_spec& B, hipStream_t stream) {
   // workgroup per CU; streaming variants get more waves.  A caller (the
   // autotuner) may override.
   int64_t cap = (BIN == 1) ? 1LL * cus : 16LL * cus;
-  if (B.grid_hint > 0) cap = B.grid_hint;
+  int32_t tuned = 0;
+  if (B.grid_hint > 0) {
+    cap = B.grid_hint;
+  } else if (BIN != 0) {
+    const int64_t key[4] = {P.n, B.height * B.width, P.dtype, 0};
+    if (tg::tune_lookup(TG_TUNE_RAY_GRID, key, tuned) && tuned > 0) cap = tuned;
+  }
   int64_t grid = nchunks < cap ? nchunks : cap;
   if (BIN == 1) {
     // packed 16-bit counters: at most kMaxRaysPerGroup rays per workgroup

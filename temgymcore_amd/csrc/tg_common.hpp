@@ -21,6 +21,9 @@ int cu_count();
 // Convert a hipError_t into TG_EHIP with a message naming `what`.
 int hip_fail(hipError_t e, const char* what);
 
+// Tuned launch choice recorded by tg_tune_set for (knob, current device, key).
+bool tune_lookup(int32_t knob, const int64_t (&key)[4], int32_t& value);
+
 }  // namespace tg
 
 #define TG_HIP_CHECK(expr)                                   \

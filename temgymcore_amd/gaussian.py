@@ -230,6 +230,10 @@ class BeamletSum:
         self.field = torch.empty(self.shape + (2,), dtype=torch.float32, device=device)
 
     def run(self, coef: torch.Tensor) -> torch.Tensor:
+        need = int(_lib.load().tg_beamlet_sum_workspace(self.n, *self.shape))
+        if need > self.workspace.numel():  # the slice count was (re)tuned
+            self.workspace = torch.empty(need, dtype=torch.uint8, device=self.field.device)
+        self.ws_bytes = max(need, 16)
         st = _lib.load().tg_beamlet_sum(coef.data_ptr(), self.n, self.shape[0], self.shape[1],
                                          self.field.data_ptr(), self.workspace.data_ptr(), self.ws_bytes,
                                          _lib.stream_ptr(self.field.device))
