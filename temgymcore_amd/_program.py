@@ -83,27 +83,27 @@ class Sym:
         self.tr = tr
         self.vid = vid
 
-    # arithmetic --------------------------------------------------------
+    # arithmetic (a FwdDual operand takes over: it carries a tangent) -------
     def __add__(self, o):
-        return self.tr.binary("add", self, o)
+        return NotImplemented if isinstance(o, FwdDual) else self.tr.binary("add", self, o)
 
     def __radd__(self, o):
         return self.tr.binary("add", o, self)
 
     def __sub__(self, o):
-        return self.tr.binary("sub", self, o)
+        return NotImplemented if isinstance(o, FwdDual) else self.tr.binary("sub", self, o)
 
     def __rsub__(self, o):
         return self.tr.binary("sub", o, self)
 
     def __mul__(self, o):
-        return self.tr.binary("mul", self, o)
+        return NotImplemented if isinstance(o, FwdDual) else self.tr.binary("mul", self, o)
 
     def __rmul__(self, o):
         return self.tr.binary("mul", o, self)
 
     def __truediv__(self, o):
-        return self.tr.binary("div", self, o)
+        return NotImplemented if isinstance(o, FwdDual) else self.tr.binary("div", self, o)
 
     def __rtruediv__(self, o):
         return self.tr.binary("div", o, self)
@@ -149,8 +149,131 @@ class Sym:
         return f"Sym(v{self.vid})"
 
 
+def _zero(t) -> bool:
+    """A structural zero tangent (emits no instructions)."""
+    return is_number(t) and not isinstance(t, bool) and t == 0
+
+
+def _tadd(a, b):
+    if _zero(a):
+        return b
+    if _zero(b):
+        return a
+    return a + b
+
+
+def _tmul(a, b):
+    if _zero(a) or _zero(b):
+        return 0.0
+    return a * b
+
+
+def _tneg(a):
+    return 0.0 if _zero(a) else -a
+
+
+class FwdDual:
+    """(value, tangent) pair for forward-mode derivatives through component
+    bodies.  Each part is a Python number or a per-ray ``Sym``; a tangent of
+    0 is a structural zero and emits no instructions.  A parameter seeded as
+    FwdDual(p, 1.0) yields d(output)/dp in the tangents (the role of
+    jax.jacobian in reference run.py:182-267)."""
+
+    __slots__ = ("v", "t")
+
+    def __init__(self, v, t=0.0):
+        self.v = v
+        self.t = t
+
+    @staticmethod
+    def lift(o) -> "FwdDual":
+        return o if isinstance(o, FwdDual) else FwdDual(o, 0.0)
+
+    def __add__(self, o):
+        o = FwdDual.lift(o)
+        return FwdDual(self.v + o.v, _tadd(self.t, o.t))
+
+    __radd__ = __add__
+
+    def __sub__(self, o):
+        o = FwdDual.lift(o)
+        return FwdDual(self.v - o.v, _tadd(self.t, _tneg(o.t)))
+
+    def __rsub__(self, o):
+        return FwdDual.lift(o) - self
+
+    def __mul__(self, o):
+        o = FwdDual.lift(o)
+        return FwdDual(self.v * o.v, _tadd(_tmul(self.t, o.v), _tmul(self.v, o.t)))
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, o):
+        o = FwdDual.lift(o)
+        q = self.v / o.v
+        # d(a/b) = da/b - (a/b) db/b
+        ta = 0.0 if _zero(self.t) else self.t / o.v
+        tb = 0.0 if _zero(o.t) else (q * o.t) / o.v
+        return FwdDual(q, _tadd(ta, _tneg(tb)))
+
+    def __rtruediv__(self, o):
+        return FwdDual.lift(o) / self
+
+    def __neg__(self):
+        return FwdDual(-self.v, _tneg(self.t))
+
+    def __pos__(self):
+        return self
+
+    def __pow__(self, p):
+        if not is_number(p) or float(p) != int(p):
+            raise NotImplementedError("only integer powers of differentiated values")
+        p = int(p)
+        if p == 0:
+            return FwdDual(1.0, 0.0)
+        if p < 0:
+            return 1.0 / (self ** (-p))
+        acc, base = None, self
+        while p:
+            if p & 1:
+                acc = base if acc is None else acc * base
+            p >>= 1
+            if p:
+                base = base * base
+        return acc
+
+    # a differentiated value must not leave the algebra silently
+    def __float__(self):
+        raise TypeError("a differentiated parameter was converted to a Python float; use _program.cos/sin/sign")
+
+    def __bool__(self):
+        raise TypeError("a differentiated parameter was used in Python control flow")
+
+    __index__ = __int__ = __float__
+
+
+def cos(v):
+    """cos of a host scalar parameter (float or FwdDual)."""
+    if isinstance(v, FwdDual):
+        if isinstance(v.v, Sym):
+            raise TypeError("cos of a per-ray value is not a ray-program operation")
+        return FwdDual(math.cos(v.v), _tmul(v.t, -math.sin(v.v)))
+    return math.cos(v)
+
+
+def sin(v):
+    """sin of a host scalar parameter (float or FwdDual)."""
+    if isinstance(v, FwdDual):
+        if isinstance(v.v, Sym):
+            raise TypeError("sin of a per-ray value is not a ray-program operation")
+        return FwdDual(math.sin(v.v), _tmul(v.t, math.cos(v.v)))
+    return math.sin(v)
+
+
 def sign(v):
     """``jnp.sign`` for numbers and per-ray values."""
+    if isinstance(v, FwdDual):
+        return FwdDual(sign(v.v), 0.0)
     if isinstance(v, Sym):
         return v.sign()
     if _is_array(v):
@@ -259,7 +382,12 @@ class Trace:
         vals = {}
         for f in dataclasses.fields(ray):
             v = getattr(ray, f.name)
-            vals[f.name] = self.lift(v) if f.name in RAY_FIELDS else v
+            if f.name not in RAY_FIELDS:
+                vals[f.name] = v
+            elif isinstance(v, FwdDual):
+                vals[f.name] = FwdDual(self.lift(v.v), v.t if is_number(v.t) else self.lift(v.t))
+            else:
+                vals[f.name] = self.lift(v)
         return _rebuild(ray, vals)
 
 
@@ -304,15 +432,10 @@ def compile_group(trace: Trace, out_vids: list[int], keep_vids: list[int], shape
     """DCE + linear-scan register allocation for the values `out_vids` (stored)
     and `keep_vids` (kept live to the end, e.g. binning positions)."""
     live_out = set(out_vids) | set(keep_vids)
-    needed = set(live_out)
-    kept = []
-    for ins in reversed(trace.instrs):
-        if ins.dst in needed:
-            kept.append(ins)
-            needed.add(ins.a)
-            needed.add(ins.b)
-    kept.reverse()
+    kept, needed = _live_instrs(trace, live_out)
     slot_vids = [v for v in trace.slot_value if v in needed]
+    if len(slot_vids) > _lib.TG_MAX_SLOT or len(set(out_vids)) > _lib.TG_MAX_SLOT:
+        raise _TooWide(len(slot_vids), len(set(out_vids)))
     dtypes = {trace.vdtype[v] for v in needed}
     dtype = torch.float64 if torch.float64 in dtypes else torch.float32
     mixed = len(dtypes) > 1
@@ -330,9 +453,7 @@ def compile_group(trace: Trace, out_vids: list[int], keep_vids: list[int], shape
 
     def alloc(v):
         if not free:
-            raise NotImplementedError(
-                f"ray program needs more than {_lib.TG_MAX_REG} live per-ray values; split the model"
-            )
+            raise _TooManyRegs(f"ray program needs more than {_lib.TG_MAX_REG} live per-ray values")
         reg[v] = free.pop()
 
     for v in slot_vids:
@@ -366,6 +487,72 @@ def compile_group(trace: Trace, out_vids: list[int], keep_vids: list[int], shape
 
 class _TooLong(Exception):
     pass
+
+
+class _TooWide(Exception):
+    """More input slots or outputs than one program holds."""
+
+
+class _TooManyRegs(NotImplementedError):
+    pass
+
+
+def _live_instrs(trace: Trace, live_out: set):
+    """Instructions needed for `live_out` (dead-code elimination), in order,
+    and every value they touch.  Values bound to slots (inputs, or results
+    materialised by an earlier program) are not recomputed."""
+    needed = set(live_out)
+    kept = []
+    for ins in reversed(trace.instrs):
+        if ins.dst in needed and ins.dst not in trace.value_slot:
+            kept.append(ins)
+            needed.add(ins.a)
+            needed.add(ins.b)
+    kept.reverse()
+    return kept, needed
+
+
+def _compile_segments(trace: Trace, out_vids: list, keep_vids: list, shape: tuple):
+    """Compile one shape group, splitting it into several programs when it
+    exceeds one program's instruction, register, slot or output limits.
+
+    A prefix program stores every value that crosses its cut to a buffer of
+    the group's shape; those buffers become input slots of the rest, which is
+    compiled the same way.  Returns ([(Program, None, keepalive)] of the
+    prefixes, Compiled of the last segment, [(Compiled, outputs)] of the
+    prefixes)."""
+    limits = (_TooLong, _TooWide, _TooManyRegs)
+    prefixes, prefix_comps = [], []
+    while True:
+        try:
+            return prefixes, compile_group(trace, out_vids, keep_vids, shape), prefix_comps
+        except limits:
+            pass
+        live_out = set(out_vids) | set(keep_vids)
+        kept, _ = _live_instrs(trace, live_out)
+        chosen = None
+        for c in range(min(len(kept) - 1, _lib.TG_MAX_INSTR), 0, -1):
+            defined = {ins.dst for ins in kept[:c]}
+            used_after = {v for ins in kept[c:] for v in (ins.a, ins.b)} | live_out
+            cross = sorted(defined & used_after)
+            if not cross or len(cross) > _lib.TG_MAX_SLOT:
+                continue
+            try:
+                chosen = (cross, compile_group(trace, cross, [], shape))
+            except limits:
+                continue
+            break
+        if chosen is None:
+            raise NotImplementedError("ray program cannot be split into programs that fit the kernel's limits")
+        cross, comp = chosen
+        prog, outs, keepalive = prepare(trace, comp, None)
+        prefixes.append((prog, None, keepalive + list(outs.values())))
+        prefix_comps.append((comp, outs))
+        for v, t in outs.items():  # later programs read these values from their buffers
+            s = len(trace.slot_tensors)
+            trace.slot_tensors.append(t)
+            trace.slot_value.append(v)
+            trace.value_slot[v] = s
 
 
 def _pad_shape(shape: tuple) -> list:
@@ -447,6 +634,7 @@ class Plan:
         self.programs = programs  # list of (Program, BinSpec | None, keepalive)
         self.out_ray = out_ray
         self.acc = acc
+        self.extra_out: list = []
 
     def launch(self, stream: int | None = None):
         lib = _lib.load()
@@ -461,13 +649,16 @@ class Plan:
 # user-facing drivers
 # --------------------------------------------------------------------------
 
-def build_plan(trace: Trace, sym_ray, bin_request=None) -> Plan:
+def build_plan(trace: Trace, sym_ray, bin_request=None, extra_rays=()) -> Plan:
     """Compile the trace into launchable programs and the output ray.
 
     bin_request: (BinSpec, x, y) with x, y the symbolic final positions, or None.
+    extra_rays: more symbolic rays of the same trace computed by the same
+    launches (e.g. derivative rays); rebuilt into ``plan.extra_out``.
     """
-    fields = {f.name: getattr(sym_ray, f.name) for f in dataclasses.fields(sym_ray)}
-    stored: dict[str, int] = {}
+    rays = [sym_ray] + list(extra_rays)
+    fields = {(i, f.name): getattr(r, f.name) for i, r in enumerate(rays) for f in dataclasses.fields(r)}
+    stored: dict = {}
     for name, v in fields.items():
         if isinstance(v, Sym):
             if v.tr is not trace:
@@ -492,7 +683,8 @@ def build_plan(trace: Trace, sym_ray, bin_request=None) -> Plan:
     for shape, vids in groups.items():
         uniq = list(dict.fromkeys(vids))
         is_bin = bin_request is not None and shape == bin_shape
-        comp = compile_group(trace, uniq, keep if is_bin else [], shape)
+        prefixes, comp, _ = _compile_segments(trace, uniq, keep if is_bin else [], shape)
+        programs.extend(prefixes)
         spec = None
         if is_bin:
             spec, bx, by = bin_request
@@ -507,15 +699,17 @@ def build_plan(trace: Trace, sym_ray, bin_request=None) -> Plan:
         prog, outs, keepalive = prepare(trace, comp, spec)
         results.update(outs)
         programs.append((prog, spec, keepalive))
-    out_vals = {}
-    for name, v in fields.items():
-        if name in direct:
-            out_vals[name] = trace.slot_tensors[trace.value_slot[direct[name]]]
-        elif name in computed:
-            out_vals[name] = results[computed[name]]
+    out_vals: list[dict] = [{} for _ in rays]
+    for (i, name), v in fields.items():
+        if (i, name) in direct:
+            out_vals[i][name] = trace.slot_tensors[trace.value_slot[direct[(i, name)]]]
+        elif (i, name) in computed:
+            out_vals[i][name] = results[computed[(i, name)]]
         else:
-            out_vals[name] = v
-    return Plan(trace, programs, _rebuild(sym_ray, out_vals))
+            out_vals[i][name] = v
+    plan = Plan(trace, programs, _rebuild(sym_ray, out_vals[0]))
+    plan.extra_out = [_rebuild(r, vals) for r, vals in zip(rays[1:], out_vals[1:])]
+    return plan
 
 
 def _materialize(trace: Trace, sym_ray, bin_request=None):

@@ -20,6 +20,7 @@ from . import _program
 from ._types import CoordsXY, ScaleYX, ShapeYX
 from .grid import Grid
 from .ray import Ray
+from .tree_utils import HasParamsMixin
 
 
 def _install_dispatch(cls) -> None:
@@ -37,7 +38,7 @@ def _install_dispatch(cls) -> None:
     cls.__call__ = __call__
 
 
-class Component:
+class Component(HasParamsMixin):
     """Base component: pure ``__call__(ray) -> Ray`` at axial position ``z``.
 
     Subclasses write the per-ray update with ordinary arithmetic on the ray's
@@ -224,7 +225,7 @@ class Rotator(Component):
     def __call__(self, ray: Ray):
         # jnp.deg2rad(a) = a * (pi / 180); cos/sin of a scalar are host f64
         angle = self.angle * (math.pi / 180)
-        c, s = math.cos(angle), math.sin(angle)
+        c, s = _program.cos(angle), _program.sin(angle)
         new_x = ray.x * c - ray.y * s
         new_y = ray.x * s + ray.y * c
         new_dx = ray.dx * c - ray.dy * s

@@ -13,10 +13,11 @@ import numpy as np
 import torch
 
 from ._program import RAY_FIELDS, _is_array
+from .tree_utils import HasParamsMixin
 
 
 @dataclasses.dataclass(frozen=True)
-class Ray:
+class Ray(HasParamsMixin):
     """Parametric ray with positions, slopes, z, and pathlength.
 
     Mirrors reference ray.py:8-45: x, y (m), dx, dy (rad, paraxial slopes),

@@ -15,9 +15,11 @@ On a scalar ray everything runs as plain Python arithmetic.
 """
 from __future__ import annotations
 
+import dataclasses
 from typing import Any, Callable, Sequence
 
 import numpy as np
+import torch
 
 from . import _lib, _program
 from .components import Component
@@ -122,3 +124,114 @@ def solve_model(ray: Ray, model: Sequence, propagator: BasePropagator = FreeSpac
     for _, jac in run_iter(ray, model, transform=jacobian_transform, propagator=propagator):
         mats.append(custom_jacobian_matrix(jac))
     return np.array(mats)
+
+
+# ---------------------------------------------------------------------------
+# Jacobians with respect to selected parameters (reference run.py:182-267)
+# ---------------------------------------------------------------------------
+def _grad_targets(input_ray, model, grad_vars):
+    """[(path, where, keys)]: `where` is "ray" or the model index; `keys`
+    lead from that object to one scalar parameter (or a ray field)."""
+    from .tree_utils import PathBuilder, get_at, scalar_leaves
+
+    out = []
+    for var in grad_vars:
+        found = []
+        if var is input_ray:
+            for f in dataclasses.fields(input_ray):
+                found.append(((input_ray, f.name), "ray", (f.name,)))
+        elif isinstance(var, PathBuilder):
+            path = var._build()
+            root, keys = path[0], tuple(path[1:])
+            if root is input_ray:
+                where = "ray"
+            else:
+                where = next((i for i, el in enumerate(model) if el is root), None)
+            if where is not None:
+                for leaf, _ in scalar_leaves(get_at(root, keys)):
+                    found.append((path + leaf, where, keys + leaf))
+        else:
+            raise TypeError(f"run_with_grads: grad_vars holds {type(var).__name__}; use x.params.<field> paths")
+        if not found:
+            what = var._build() if isinstance(var, PathBuilder) else var
+            raise RuntimeError(f"Cannot find {what} in parameters")
+        out += found
+    return out
+
+
+def _seeded(value, is_ray_field: bool):
+    if isinstance(value, torch.Tensor) and not is_ray_field:
+        if value.numel() != 1:
+            raise NotImplementedError("run_with_grads: array-valued model parameters (differentiate scalars)")
+        value = value.item()
+    return _program.FwdDual(value, 1.0)
+
+
+def _split(out_ray, like):
+    """(value ray, tangent ray) of an output ray whose fields may be FwdDual."""
+    vals, tans = {}, {}
+    for f in dataclasses.fields(out_ray):
+        v = getattr(out_ray, f.name)
+        if isinstance(v, _program.FwdDual):
+            vals[f.name], tans[f.name] = v.v, v.t
+        else:
+            vals[f.name], tans[f.name] = v, 0.0
+    return _program._rebuild(like, vals), _program._rebuild(like, tans)
+
+
+def _shape_tangents(value_ray, tan_ray):
+    """Tangents shaped like the values: uniform tangents of tensor fields
+    become full tensors (the reference returns arrays)."""
+    out = {}
+    for f in dataclasses.fields(value_ray):
+        v, t = getattr(value_ray, f.name), getattr(tan_ray, f.name)
+        if isinstance(v, torch.Tensor) and not isinstance(t, torch.Tensor):
+            out[f.name] = torch.full_like(v, float(t))
+        else:
+            out[f.name] = t
+    return _program._rebuild(value_ray, out)
+
+
+def run_with_grads(input_ray: Ray, model: Sequence, grad_vars: Sequence,
+                   propagator: BasePropagator = FreeSpaceParaxial()):
+    """Run the model and the Jacobians of the output ray with respect to the
+    selected variables (reference run.py:182-267).
+
+    grad_vars: paths built with ``.params`` -- ``model[i].params.focal_length``,
+    ``input_ray.params.x``, a non-leaf such as ``descanner.params.descan_error``
+    (every scalar under it), or the input ray itself (every field).
+
+    Returns (value, grads): the output Ray, and a dict from each variable's
+    path (root object, keys...) to a Ray of d(output field)/d(variable).
+    Derivatives are forward-mode: for vectorised rays every variable costs one
+    fused launch that computes the outputs and their derivatives together.
+    A ray-field variable of a vectorised ray is differentiated per ray (the
+    diagonal of the reference's [N, N] Jacobian); model parameters must be
+    scalars.  Raises RuntimeError for a variable that is not in the model.
+    """
+    from .tree_utils import replace_at
+
+    model = list(model)
+    targets = _grad_targets(input_ray, model, grad_vars)
+    value = run_to_end(input_ray, model, propagator)
+    grads = {}
+    for path, where, keys in targets:
+        if where == "ray":
+            ray_d = replace_at(input_ray, keys, lambda v: _seeded(v, True))
+            model_d = model
+        else:
+            ray_d = input_ray
+            model_d = list(model)
+            model_d[where] = replace_at(model[where], keys, lambda v: _seeded(v, False))
+        if _program.is_device_ray(input_ray):
+            trace = _program.Trace(_lib.require_device())
+            out = _trace_model(trace.symbolize(ray_d), model_d, propagator)
+            val_ray, tan_ray = _split(out, input_ray)
+            plan = _program.build_plan(trace, val_ray, extra_rays=[tan_ray])
+            value_i = plan.launch()
+            grads[path] = _shape_tangents(value_i, plan.extra_out[0])
+        else:
+            out = _trace_model(ray_d, model_d, propagator)
+            _, tan_ray = _split(out, input_ray)
+            grads[path] = tan_ray
+    return value, grads

@@ -12,6 +12,7 @@ import torch
 
 from ._types import CoordsXY
 from .ray import Ray
+from .tree_utils import HasParamsMixin
 
 
 def multi_cumsum_inplace(values: np.ndarray, partitions: np.ndarray, start: float) -> None:
@@ -73,7 +74,7 @@ def fibonacci_spiral(nb_samples: int, radius: float, alpha=2):
     return rr * np.cos(phi), rr * np.sin(phi)
 
 
-class Source:
+class Source(HasParamsMixin):
     """Base class of ray sources; calling one is a no-op (reference source.py:11-79)."""
 
     z: float

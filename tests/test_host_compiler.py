@@ -286,3 +286,48 @@ def test_struct_layout_matches_header(tmp_path):
     subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     assert got == expect
+
+
+def _run_host_split(ray, model):
+    """Like run_host, through the splitting compiler: every prefix program is
+    interpreted into its buffers before the next one runs."""
+    tr = _program.Trace(torch.device("cpu"))
+    out = _trace_model(tr.symbolize(ray), model, FreeSpaceParaxial())
+    names = [n for n in ("x", "y", "dx", "dy", "pathlength") if isinstance(getattr(out, n), _program.Sym)]
+    vids = [getattr(out, n).vid for n in names]
+    shape = _program._shape_of(tr, vids[0])
+    prefixes, last, prefix_comps = _program._compile_segments(tr, vids, [], shape)
+    for comp, outs in prefix_comps:
+        vals = interpret(tr, comp)
+        for v, t in outs.items():
+            t.copy_(torch.from_numpy(vals[v]))
+    vals = interpret(tr, last)
+    return {n: vals[v] for n, v in zip(names, vids)}, len(prefixes)
+
+
+def test_long_model_is_split_into_programs_bit_exact():
+    """A model longer than one program (96 instructions) runs as several
+    programs that hand values over through buffers, bit-exact to the oracle."""
+    x, y, dx, dy = O.plain_workload(4000, seed=5)
+    ray = Ray(x=torch.from_numpy(x), y=torch.from_numpy(y), dx=torch.from_numpy(dx), dy=torch.from_numpy(dy),
+              z=0.0, pathlength=0.0)
+    model, omodel = [], []
+    for i in range(14):
+        z = 0.05 * (i + 1)
+        if i % 2:
+            model.append(Lens(z=z, focal_length=0.7 + 0.01 * i))
+            omodel.append(("lens", z, 0.7 + 0.01 * i))
+        else:
+            model.append(Deflector(z=z, def_x=1e-3 * i, def_y=-2e-4))
+            omodel.append(("deflector", z, 1e-3 * i, -2e-4))
+    model.append(Detector(z=1.0, pixel_size=(0.01, 0.01), shape=(8, 8)))
+    omodel.append(("detector", 1.0))
+    tr = _program.Trace(torch.device("cpu"))
+    out = _trace_model(tr.symbolize(ray), model, FreeSpaceParaxial())
+    with pytest.raises((_program._TooLong, _program._TooManyRegs)):
+        _program.compile_group(tr, [out.x.vid, out.y.vid, out.dx.vid, out.dy.vid, out.pathlength.vid], [], (4000,))
+    res, n_prefix = _run_host_split(ray, model)
+    assert n_prefix >= 1
+    ref = O.run_to_end(O.make_ray(x, y, dx, dy), omodel)
+    for k in ("x", "y", "dx", "dy", "pathlength"):
+        np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
