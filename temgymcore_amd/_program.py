@@ -278,6 +278,8 @@ def sign(v):
         return v.sign()
     if _is_array(v):
         return torch.sign(v) if isinstance(v, torch.Tensor) else np.sign(v)
+    if not is_number(v) and hasattr(v, "sign"):  # transfer.Dual (5x5 Jacobians)
+        return v.sign()
     if v != v or v == 0:
         return v
     return 1.0 if v > 0 else -1.0

@@ -11,7 +11,9 @@ the reference's uncut sum.  ``cutoff=4.0`` is the "4 sigma" cut.
 The per-beamlet 5x5 ray-transfer matrix comes from ``transfer.jacobian`` of
 ``run_to_end`` over the model (reference gaussian.py:234-238 takes
 ``jax.jacobian``), evaluated once per distinct input-plane z -- exact for
-models built from the affine elements (every element except Biprism).
+models built from the affine elements.  With a Biprism the matrix depends
+on each beamlet's central ray; it is then taken per beamlet by forward-mode
+tangents on the GPU (``_abcd_traced``).
 
 Both images are differentiable: when a GaussianRay field (a torch tensor with
 requires_grad) or an explicit ``abcd`` tensor requires a gradient, the field
@@ -185,13 +187,44 @@ def _pix2m(grid) -> np.ndarray:
     return np.array([T[0, 0], T[0, 1], T[0, 2], T[1, 0], T[1, 1], T[1, 2]], dtype=np.float64)
 
 
+_JAC_FIELDS = ("x", "y", "dx", "dy", "_one")
+
+
+def _abcd_traced(rays: GaussianRay, model, device, n: int) -> torch.Tensor:
+    """Per-beamlet 5x5 Jacobians of run_to_end at every beamlet's central ray
+    (reference gaussian.py:234-238: vmap(jacobian(run_to_end))), for models
+    whose matrix depends on the ray (Biprism).  Forward-mode tangents: one
+    fused launch per input field gives one column of every beamlet's matrix."""
+    from .run import run_with_grads
+
+    def col(v):
+        t = v if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v, dtype=np.float64))
+        t = t.detach().to(device=device, dtype=torch.float64).reshape(-1)
+        return t.expand(n).contiguous() if t.numel() == 1 else t.reshape(n).contiguous()
+
+    r = rays.to_ray()
+    cr = Ray(x=col(r.x), y=col(r.y), dx=col(r.dx), dy=col(r.dy), z=col(r.z), pathlength=col(r.pathlength),
+             _one=torch.ones(n, dtype=torch.float64, device=device))
+    _, grads = run_with_grads(cr, list(model), [getattr(cr.params, f) for f in _JAC_FIELDS])
+    J = torch.empty((n, 5, 5), dtype=torch.float64, device=device)
+    for j, fin in enumerate(_JAC_FIELDS):
+        g = grads[(cr, fin)]
+        for i, fout in enumerate(_JAC_FIELDS):
+            v = getattr(g, fout)
+            J[:, i, j] = v.reshape(-1) if isinstance(v, torch.Tensor) and v.numel() == n else float(v)
+    return J
+
+
 def _abcd_per_beamlet(rays: GaussianRay, model, device, n: int):
-    """(abcd tensor [m, 5, 5] f64, stride) of the model for each beamlet's z."""
+    """(abcd tensor [m, 5, 5] f64, stride) of the model for each beamlet's z
+    (or for each beamlet, when the model has ray-dependent elements)."""
     from .components import Biprism
     from .transfer import ray_matrix
 
     if any(isinstance(c, Biprism) for c in model):
-        raise NotImplementedError("make_gaussian_image: per-beamlet ABCD of nonlinear elements (Biprism)")
+        if n == 0:
+            return torch.zeros((1, 5, 5), dtype=torch.float64, device=device), 0
+        return _abcd_traced(rays, model, device, n), 25
     z = rays.z if isinstance(rays.z, torch.Tensor) else torch.as_tensor(np.asarray(rays.z, dtype=np.float64))
     z = z.reshape(-1).to(torch.float64)
     if n == 0:

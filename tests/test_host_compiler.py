@@ -331,3 +331,27 @@ def test_long_model_is_split_into_programs_bit_exact():
     ref = O.run_to_end(O.make_ray(x, y, dx, dy), omodel)
     for k in ("x", "y", "dx", "dy", "pathlength"):
         np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
+
+
+def test_ray_matrix_through_biprism_vs_oracle_differences():
+    """The 5x5 ABCD of a model with a Biprism depends on the side of the
+    biprism the central ray passes (reference components.py:553-559)."""
+    from temgymcore_amd.components import Biprism
+
+    model = [Lens(z=2e-4, focal_length=5e-4), Biprism(z=4e-4, def_x=1e-5),
+             Detector(z=1e-3, pixel_size=(3e-8, 3e-8), shape=(8, 8))]
+    model_t = [("lens", 2e-4, 5e-4), ("biprism", 4e-4, 1e-5), ("detector", 1e-3)]
+    for x0 in (1e-7, -1e-7):
+        J = np.asarray(ray_matrix(model, Ray(x0, 2e-8, 1e-5, 0.0, 0.0, 0.0)))
+        base = np.array([x0, 2e-8, 1e-5, 0.0, 1.0])
+        ref = np.zeros((5, 5))
+        for j in range(5):
+            outs = []
+            for s in (1, -1):
+                v = base.copy()
+                v[j] += s * 1e-10
+                r = O.run_to_end(O.make_ray(v[0], v[1], v[2], v[3], 0.0, 0.0, v[4]), model_t)
+                outs.append(np.array([r["x"], r["y"], r["dx"], r["dy"], r["one"]]))
+            ref[:, j] = (outs[0] - outs[1]) / 2e-10
+        np.testing.assert_allclose(J, ref, rtol=1e-6, atol=1e-9)
+        assert np.sign(J[2, 4]) == np.sign(x0)
