@@ -31,7 +31,7 @@ extern "C" {
 const char* tg_last_error(void);
 
 /* ABI version; bumped whenever a struct layout below changes. */
-int32_t tg_abi_version(void);  /* currently 3 */
+int32_t tg_abi_version(void);  /* currently 4 */
 
 /* ======================================================================== *
  * Ray branch: per-ray register programs
@@ -140,8 +140,10 @@ int32_t tg_device_cu_count(void);
  *   TG_TUNE_RAY_GRID       key {n, H*W, dtype, 0}: workgroups of a binning
  *                          tg_ray_program_run whose bin->grid_hint is 0
  *   TG_TUNE_FIT_SLICE      key {n_inst, n_gauss, n_pix, 0}: target Gaussians
- *                          per forward slice in tg_gauss_fit_plan, 16..4096 */
-enum tg_tune_knob { TG_TUNE_BEAMLET_SPLITS = 1, TG_TUNE_RAY_GRID = 2, TG_TUNE_FIT_SLICE = 3 };
+ *                          per forward slice in tg_gauss_fit_plan, 16..4096
+ *   TG_TUNE_STEM_FIT       key {S, J, n_cam, 0}: tg_stem_fit launch variant
+ *                          (tg_stem_fit_variant) */
+enum tg_tune_knob { TG_TUNE_BEAMLET_SPLITS = 1, TG_TUNE_RAY_GRID = 2, TG_TUNE_FIT_SLICE = 3, TG_TUNE_STEM_FIT = 4 };
 int tg_tune_set(int32_t knob, const int64_t* key, int32_t value);
 /* 1 and *value set when an entry exists, 0 when not, < 0 on error. */
 int tg_tune_get(int32_t knob, const int64_t* key, int32_t* value);
@@ -196,11 +198,18 @@ int tg_stem_forward(const tg_stem_geometry* g, const double* params, const float
 
 /* Descan / geometry fit, forward and adjoint in one pass over the targets.
  *   targets  [n_cam, S, J, 2] f32 target detector coordinates
- *   partials [18 + n_cam + 1, S] f64 workspace (per-scan-position terms,
- *            one row per output entry)
+ *   workspace tg_stem_fit_workspace(g) bytes of f64: starts with the
+ *            [18 + n_cam + 1, S] per-scan-position terms (one row per
+ *            output entry), then [S, n_cam, 7] per-camera moments
  *   out      [18 + n_cam + 1] f64: d loss / d params, then the loss
- * loss = sum over (c, s, j) of |detector(x, y) - target|^2. */
-int tg_stem_fit(const tg_stem_geometry* g, const double* params, const float* targets, double* partials,
+ * loss = sum over (c, s, j) of |detector(x, y) - target|^2.
+ * Two launch variants give identical results: TG_STEM_FIT_SCAN (one
+ * workgroup per scan position; the default when S >= 4 x the CU count) and
+ * TG_STEM_FIT_CAMERA (one per scan position and camera length; the default
+ * for smaller scans); the tuning knob TG_TUNE_STEM_FIT overrides. */
+enum tg_stem_fit_variant { TG_STEM_FIT_SCAN = 1, TG_STEM_FIT_CAMERA = 2 };
+int64_t tg_stem_fit_workspace(const tg_stem_geometry* g);
+int tg_stem_fit(const tg_stem_geometry* g, const double* params, const float* targets, double* workspace,
                 double* out, void* stream);
 
 /* ======================================================================== *

@@ -15,7 +15,7 @@ import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtemgym_hip.so")
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 TG_MAX_INSTR = 96
 TG_MAX_REG = 16

@@ -525,6 +525,16 @@ def _compile_segments(trace: Trace, out_vids: list, keep_vids: list, shape: tupl
     prefixes)."""
     limits = (_TooLong, _TooWide, _TooManyRegs)
     prefixes, prefix_comps = [], []
+    out_vids = list(dict.fromkeys(out_vids))
+    while len(out_vids) > _lib.TG_MAX_SLOT:
+        # more outputs than one program stores: the first ones get programs
+        # of their own, whose results later programs read instead of recomputing
+        head, out_vids = out_vids[:_lib.TG_MAX_SLOT], out_vids[_lib.TG_MAX_SLOT:]
+        p, comp, pc = _compile_segments(trace, head, [], shape)
+        prog, outs, keepalive = prepare(trace, comp, None)
+        prefixes += p + [(prog, None, keepalive + list(outs.values()))]
+        prefix_comps += pc + [(comp, outs)]
+        _bind_slots(trace, outs)
     while True:
         try:
             return prefixes, compile_group(trace, out_vids, keep_vids, shape), prefix_comps
@@ -550,11 +560,16 @@ def _compile_segments(trace: Trace, out_vids: list, keep_vids: list, shape: tupl
         prog, outs, keepalive = prepare(trace, comp, None)
         prefixes.append((prog, None, keepalive + list(outs.values())))
         prefix_comps.append((comp, outs))
-        for v, t in outs.items():  # later programs read these values from their buffers
-            s = len(trace.slot_tensors)
-            trace.slot_tensors.append(t)
-            trace.slot_value.append(v)
-            trace.value_slot[v] = s
+        _bind_slots(trace, outs)
+
+
+def _bind_slots(trace: Trace, outs: dict):
+    """Later programs read these values from their buffers."""
+    for v, t in outs.items():
+        s = len(trace.slot_tensors)
+        trace.slot_tensors.append(t)
+        trace.slot_value.append(v)
+        trace.value_slot[v] = s
 
 
 def _pad_shape(shape: tuple) -> list:
@@ -685,8 +700,10 @@ def build_plan(trace: Trace, sym_ray, bin_request=None, extra_rays=()) -> Plan:
     for shape, vids in groups.items():
         uniq = list(dict.fromkeys(vids))
         is_bin = bin_request is not None and shape == bin_shape
-        prefixes, comp, _ = _compile_segments(trace, uniq, keep if is_bin else [], shape)
+        prefixes, comp, prefix_comps = _compile_segments(trace, uniq, keep if is_bin else [], shape)
         programs.extend(prefixes)
+        for _, outs in prefix_comps:  # outputs stored by a prefix program
+            results.update(outs)
         spec = None
         if is_bin:
             spec, bx, by = bin_request

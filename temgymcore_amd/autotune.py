@@ -11,6 +11,8 @@ Knobs (include/temgym_hip.h, ``tg_tune_knob``):
   BEAMLET_SPLITS  beamlet slices per detector tile of ``tg_beamlet_sum``
   RAY_GRID        workgroups of a binning ``tg_ray_program_run``
   FIT_SLICE       Gaussians per forward slice of the batched fits
+  STEM_FIT        launch variant of the descan fit (per scan position, or
+                  per scan position and camera length)
 """
 from __future__ import annotations
 
@@ -21,8 +23,9 @@ import torch
 
 from . import _lib
 
-BEAMLET_SPLITS, RAY_GRID, FIT_SLICE = 1, 2, 3
-_NAMES = {BEAMLET_SPLITS: "beamlet_splits", RAY_GRID: "ray_grid", FIT_SLICE: "fit_slice"}
+BEAMLET_SPLITS, RAY_GRID, FIT_SLICE, STEM_FIT = 1, 2, 3, 4
+_NAMES = {BEAMLET_SPLITS: "beamlet_splits", RAY_GRID: "ray_grid", FIT_SLICE: "fit_slice", STEM_FIT: "stem_fit"}
+STEM_FIT_SCAN, STEM_FIT_CAMERA = 1, 2
 
 
 def _key(key):
@@ -92,6 +95,14 @@ def tune_run_to_image(plan, candidates=None, reps: int = 5):
         candidates = [cus // 2, cus, 2 * cus, 4 * cus]
     key = (prog.n, bs.height * bs.width, prog.dtype)
     return _pick(RAY_GRID, key, [c for c in candidates if c > 0], lambda: plan.launch(), reps)
+
+
+def tune_stem_fit(ws, params: torch.Tensor, targets: torch.Tensor, candidates=(STEM_FIT_SCAN, STEM_FIT_CAMERA),
+                  reps: int = 5):
+    """Launch variant of tg_stem_fit for this geometry (ws: STEMFitWorkspace)."""
+    g = ws.geometry
+    key = (g.scan_h * g.scan_w, g.n_side * g.n_side, g.n_cam)
+    return _pick(STEM_FIT, key, list(candidates), lambda: ws.run(params, targets), reps)
 
 
 def tune_gauss_fit(grids, counts, params: torch.Tensor, target: torch.Tensor, candidates=(128, 256, 512, 1024),

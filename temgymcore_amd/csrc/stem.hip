@@ -117,12 +117,19 @@ __global__ __launch_bounds__(kThreads) void stem_forward_kernel(const tg_stem_ge
   }
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kThreads * kFwdRays;
+  // ray j = (row iy, column ix), stepped by kThreads without dividing
+  const int n = G.n_side, drow = kThreads / n, dcol = kThreads % n;
+  const int j0 = (int)(base + threadIdx.x);  // J <= 2^20
+  int iy = j0 / n, ix = j0 - (j0 / n) * n;
 #pragma unroll
-  for (int u = 0; u < kFwdRays; ++u) {
+  for (int u = 0; u < kFwdRays; ++u, ix += dcol, iy += drow) {
+    if (ix >= n) {
+      ix -= n;
+      ++iy;
+    }
     const int64_t j = base + (int64_t)u * kThreads + threadIdx.x;
     if (j >= J) continue;
-    const int jj = (int)j;  // J <= 2^20
-    const float tx = th[jj % G.n_side], ty = th[jj / G.n_side];
+    const float tx = th[ix], ty = th[iy];
     float v = 0.0f;
     if (values != nullptr || image4d != nullptr) {
       // sample plane position in f64 (scan offsets are ~1e-9 m)
@@ -181,11 +188,12 @@ __device__ __forceinline__ void load_tile(const f4* __restrict__ T4, int64_t S, 
   }
 }
 
-// Per-scan-position gradient terms from the reduced moments M[c][k]
-// (thread t < np writes row t of the workspace column s).
+// Per-scan-position gradient terms from the per-camera moments M[c][k]
+// (FP64 sums of the wavefront partials, waves in order); thread t < np
+// writes row t of the workspace column s.
 __device__ __forceinline__ void fit_epilogue(const tg_stem_geometry& G, const double* __restrict__ P,
-                                             const ScanPos& p, const double (&mom)[TG_STEM_MAX_CAM][kWaves][kMom],
-                                             int64_t s, int64_t S, double* __restrict__ partials) {
+                                             const ScanPos& p, const double (&M)[TG_STEM_MAX_CAM][kMom], int64_t s,
+                                             int64_t S, double* __restrict__ partials) {
   const int np = 18 + G.n_cam + 1;
   const int t = threadIdx.x;
   if (t >= np) return;
@@ -194,18 +202,10 @@ __device__ __forceinline__ void fit_epilogue(const tg_stem_geometry& G, const do
   const double vv[4] = {p.spx, p.spy, p.stx, p.sty};
   double g = 0.0;
   for (int c = 0; c < G.n_cam; ++c) {
-    double M[kMom];
-#pragma unroll
-    for (int k = 0; k < kMom; ++k) {
-      double acc = 0.0;
-#pragma unroll
-      for (int w = 0; w < kWaves; ++w) acc += mom[c][w][k];
-      M[k] = acc;
-    }
     const double L = P[18 + c];
     if (t < 16) {
       const int r = t >> 2, k = t & 3;
-      const double m0 = (r == 0 || r == 2) ? M[0] : M[3];
+      const double m0 = (r == 0 || r == 2) ? M[c][0] : M[c][3];
       const double scale = (r >= 2) ? L : 1.0;
       g += 2.0 * vv[k] * scale * m0;
     } else if (t == 16 || t == 17) {
@@ -222,113 +222,216 @@ __device__ __forceinline__ void fit_epilogue(const tg_stem_geometry& G, const do
         dsx = p.spy * (kPi / 180.0);
         dsy = -p.spx * (kPi / 180.0);
       }
-      g += 2.0 * ((dbx_dx * dsx + dbx_dy * dsy) * M[0] + (dby_dx * dsx + dby_dy * dsy) * M[3]);
+      g += 2.0 * ((dbx_dx * dsx + dbx_dy * dsy) * M[c][0] + (dby_dx * dsx + dby_dy * dsy) * M[c][3]);
     } else if (t == 18 + c) {
       const double dbx_dl = row_dot(A, 2, p) + G.offsets[2];
       const double dby_dl = row_dot(A, 3, p) + G.offsets[3];
-      g += 2.0 * (M[1] + dbx_dl * M[0] + M[5] + dby_dl * M[3]);
+      g += 2.0 * (M[c][1] + dbx_dl * M[c][0] + M[c][5] + dby_dl * M[c][3]);
     } else if (t == np - 1) {
-      g += M[6];
+      g += M[c][6];
     }
   }
   partials[(int64_t)t * S + s] = g;
 }
 
-// One workgroup per scan position.  The targets are streamed in tiles of
-// kPerLane float4 (ray pairs) per lane, double-buffered across tiles and
-// camera lengths, so the per-camera wavefront reductions never leave the load
-// pipeline empty.  (A persistent variant that also prefetched across scan
-// positions needed 140 VGPRs, fell to 3 waves per SIMD and was slower.)
+// Camera-length constants of the residual r = th a + b - t as double-float
+// splits: r = (fma(th, a_hi, -t) + b_hi) + fma(th, a_lo, b_lo) is exact to
+// about one ulp of r itself (fma(th, a_hi, -t) rounds once), not to one ulp
+// of the ~1e-4 m detector coordinate.
+struct CamConst {
+  float a_hi, a_lo, bx_hi, bx_lo, by_hi, by_lo;
+};
+
+__device__ __forceinline__ CamConst cam_const(const tg_stem_geometry& G, const double* __restrict__ P,
+                                              const ScanPos& p, int c) {
+  const double L = P[18 + c];
+  double bx, by;
+  offsets_at(G, P, p, L, bx, by);
+  const double ad = G.defocus + L;
+  CamConst k;
+  k.a_hi = (float)ad;
+  k.a_lo = (float)(ad - (double)k.a_hi);
+  k.bx_hi = (float)bx;
+  k.bx_lo = (float)(bx - (double)k.bx_hi);
+  k.by_hi = (float)by;
+  k.by_lo = (float)(by - (double)k.by_hi);
+  return k;
+}
+
+// Adds one tile's residual moments to m[7] (per lane, FP32).
+__device__ __forceinline__ void accumulate_tile(const tg_stem_geometry& G, const float* __restrict__ th,
+                                                const CamConst& K, const f4 (&cur)[kPerLane], int64_t tile,
+                                                int64_t npair, float (&m)[kMom]) {
+  const int half = G.n_side >> 1, drow = kThreads / half, dcol = kThreads % half;
+  // ray pair q = (row iy, columns 2*hc, 2*hc+1); stepped without division
+  const int64_t q0 = tile * kTile + threadIdx.x;
+  int iy = (int)(q0 / half), hc = (int)(q0 % half);
+  if (dcol == 0 && (tile + 1) * kTile <= npair) {
+    // full tile and a column count that divides the workgroup: this lane's
+    // two columns are fixed, rows advance by drow per pair
+    const float tx0 = th[2 * hc], tx1 = th[2 * hc + 1];
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) {
+      const f4 t = cur[k];
+      const float ty = th[iy + k * drow];
+      const float rx0 = (fmaf(tx0, K.a_hi, -t.x) + K.bx_hi) + fmaf(tx0, K.a_lo, K.bx_lo);
+      const float ry0 = (fmaf(ty, K.a_hi, -t.y) + K.by_hi) + fmaf(ty, K.a_lo, K.by_lo);
+      const float rx1 = (fmaf(tx1, K.a_hi, -t.z) + K.bx_hi) + fmaf(tx1, K.a_lo, K.bx_lo);
+      const float ry1 = (fmaf(ty, K.a_hi, -t.w) + K.by_hi) + fmaf(ty, K.a_lo, K.by_lo);
+      m[0] += rx0 + rx1;
+      m[1] = fmaf(rx0, tx0, fmaf(rx1, tx1, m[1]));
+      m[2] = fmaf(rx0 + rx1, ty, m[2]);
+      m[3] += ry0 + ry1;
+      m[4] = fmaf(ry0, tx0, fmaf(ry1, tx1, m[4]));
+      m[5] = fmaf(ry0 + ry1, ty, m[5]);
+      m[6] = fmaf(rx0, rx0, fmaf(ry0, ry0, fmaf(rx1, rx1, fmaf(ry1, ry1, m[6]))));
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < kPerLane; ++k) {
+    if (q0 + (int64_t)k * kThreads < npair) {
+      const f4 t = cur[k];
+      const float tx0 = th[2 * hc], tx1 = th[2 * hc + 1], ty = th[iy];
+      const float rx0 = (fmaf(tx0, K.a_hi, -t.x) + K.bx_hi) + fmaf(tx0, K.a_lo, K.bx_lo);
+      const float ry0 = (fmaf(ty, K.a_hi, -t.y) + K.by_hi) + fmaf(ty, K.a_lo, K.by_lo);
+      const float rx1 = (fmaf(tx1, K.a_hi, -t.z) + K.bx_hi) + fmaf(tx1, K.a_lo, K.bx_lo);
+      const float ry1 = (fmaf(ty, K.a_hi, -t.w) + K.by_hi) + fmaf(ty, K.a_lo, K.by_lo);
+      m[0] += rx0 + rx1;
+      m[1] = fmaf(rx0, tx0, fmaf(rx1, tx1, m[1]));
+      m[2] = fmaf(rx0 + rx1, ty, m[2]);
+      m[3] += ry0 + ry1;
+      m[4] = fmaf(ry0, tx0, fmaf(ry1, tx1, m[4]));
+      m[5] = fmaf(ry0 + ry1, ty, m[5]);
+      m[6] = fmaf(rx0, rx0, fmaf(ry0, ry0, fmaf(rx1, rx1, fmaf(ry1, ry1, m[6]))));
+    }
+    hc += dcol;
+    iy += drow;
+    if (hc >= half) {
+      hc -= half;
+      ++iy;
+    }
+  }
+}
+
+// Wavefront sums of m[7] into wm[wave][k] (lane 0 writes).
+__device__ __forceinline__ void wave_moments(const float (&m)[kMom], double (&wm)[kWaves][kMom]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < kMom; ++k) {
+    const float v = wave_sum(m[k]);
+    if (lane == 0) wm[wave][k] = v;
+  }
+}
+
+__device__ __forceinline__ void fill_slopes(const tg_stem_geometry& G, float* th) {
+  for (int k = threadIdx.x; k < G.n_side; k += kThreads) th[k] = slope_of(G, k);
+}
+
+// Variant "scan": one workgroup per scan position streams the tiles of every
+// camera length, double-buffered across tiles and cameras.  (A persistent
+// variant that also prefetched across scan positions needed 140 VGPRs, fell
+// to 3 waves per SIMD and was slower.)
 __global__ __launch_bounds__(kThreads) void stem_fit_kernel(const tg_stem_geometry G, const double* __restrict__ P,
                                                             const float* __restrict__ T,
                                                             double* __restrict__ partials) {
   __shared__ float th[kMaxSide];
-  __shared__ double mom[TG_STEM_MAX_CAM][kWaves][kMom];
+  __shared__ double wm[TG_STEM_MAX_CAM][kWaves][kMom];
+  __shared__ double M[TG_STEM_MAX_CAM][kMom];
   const int64_t S = (int64_t)G.scan_h * G.scan_w;
   const int64_t J = (int64_t)G.n_side * G.n_side;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k = threadIdx.x; k < G.n_side; k += kThreads) th[k] = slope_of(G, k);
+  fill_slopes(G, th);
   __syncthreads();
   const int64_t npair = J >> 1;  // two rays per float4 of targets (n_side even)
-  const int half = G.n_side >> 1, drow = kThreads / half, dcol = kThreads % half;
   const int64_t ntile = (npair + kTile - 1) / kTile;
   const int64_t nit = (int64_t)G.n_cam * ntile;
   const f4* __restrict__ T4 = (const f4*)T;
   f4 cur[kPerLane], nxt[kPerLane];
   const int64_t s = blockIdx.x;
   load_tile(T4, S, s, npair, ntile, 0, cur);
-  {
-    const ScanPos p = scan_position(G, P, s);
-    float a_hi = 0.f, a_lo = 0.f, bx_hi = 0.f, bx_lo = 0.f, by_hi = 0.f, by_lo = 0.f;
-    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f, m4 = 0.f, m5 = 0.f, m6 = 0.f;
-    for (int64_t it = 0; it < nit; ++it) {
-      if (it + 1 < nit) load_tile(T4, S, s, npair, ntile, it + 1, nxt);
-      const int c = (int)(it / ntile);
-      const int64_t tile = it - (int64_t)c * ntile;
-      if (tile == 0) {
-        const double L = P[18 + c];
-        double bx, by;
-        offsets_at(G, P, p, L, bx, by);
-        // double-float split of the slope scale and offsets: the residual
-        //   r = th*a + b - t = (fma(th, a_hi, -t) + b_hi) + fma(th, a_lo, b_lo)
-        // is exact to about one ulp of r itself (fma(th, a_hi, -t) rounds once),
-        // not to one ulp of the ~1e-4 m detector coordinate.
-        const double ad = G.defocus + L;
-        a_hi = (float)ad;
-        a_lo = (float)(ad - (double)a_hi);
-        bx_hi = (float)bx;
-        bx_lo = (float)(bx - (double)bx_hi);
-        by_hi = (float)by;
-        by_lo = (float)(by - (double)by_hi);
-        m0 = m1 = m2 = m3 = m4 = m5 = m6 = 0.f;
-      }
-      // ray pair q = (row iy, columns 2*hc, 2*hc+1); stepped without division
-      const int64_t q0 = tile * kTile + threadIdx.x;
-      int iy = (int)(q0 / half), hc = (int)(q0 % half);
+  const ScanPos p = scan_position(G, P, s);
+  CamConst K = {};
+  float m[kMom];
+  for (int64_t it = 0; it < nit; ++it) {
+    if (it + 1 < nit) load_tile(T4, S, s, npair, ntile, it + 1, nxt);
+    const int c = (int)(it / ntile);
+    const int64_t tile = it - (int64_t)c * ntile;
+    if (tile == 0) {
+      K = cam_const(G, P, p, c);
 #pragma unroll
-      for (int k = 0; k < kPerLane; ++k) {
-        const bool valid = q0 + (int64_t)k * kThreads < npair;
-        if (valid) {
-          const f4 t = cur[k];
-          const float tx0 = th[2 * hc], tx1 = th[2 * hc + 1], ty = th[iy];
-          const float rx0 = (fmaf(tx0, a_hi, -t.x) + bx_hi) + fmaf(tx0, a_lo, bx_lo);
-          const float ry0 = (fmaf(ty, a_hi, -t.y) + by_hi) + fmaf(ty, a_lo, by_lo);
-          const float rx1 = (fmaf(tx1, a_hi, -t.z) + bx_hi) + fmaf(tx1, a_lo, bx_lo);
-          const float ry1 = (fmaf(ty, a_hi, -t.w) + by_hi) + fmaf(ty, a_lo, by_lo);
-          m0 += rx0 + rx1;
-          m1 = fmaf(rx0, tx0, fmaf(rx1, tx1, m1));
-          m2 = fmaf(rx0 + rx1, ty, m2);
-          m3 += ry0 + ry1;
-          m4 = fmaf(ry0, tx0, fmaf(ry1, tx1, m4));
-          m5 = fmaf(ry0 + ry1, ty, m5);
-          m6 = fmaf(rx0, rx0, fmaf(ry0, ry0, fmaf(rx1, rx1, fmaf(ry1, ry1, m6))));
-        }
-        hc += dcol;
-        iy += drow;
-        if (hc >= half) {
-          hc -= half;
-          ++iy;
-        }
-      }
-      if (tile == ntile - 1) {
-        const float w0 = wave_sum(m0), w1 = wave_sum(m1), w2 = wave_sum(m2), w3 = wave_sum(m3);
-        const float w4 = wave_sum(m4), w5 = wave_sum(m5), w6 = wave_sum(m6);
-        if (lane == 0) {
-          mom[c][wave][0] = w0;
-          mom[c][wave][1] = w1;
-          mom[c][wave][2] = w2;
-          mom[c][wave][3] = w3;
-          mom[c][wave][4] = w4;
-          mom[c][wave][5] = w5;
-          mom[c][wave][6] = w6;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < kPerLane; ++k) cur[k] = nxt[k];
+      for (int k = 0; k < kMom; ++k) m[k] = 0.f;
     }
-    __syncthreads();
-    fit_epilogue(G, P, p, mom, s, S, partials);
+    accumulate_tile(G, th, K, cur, tile, npair, m);
+    if (tile == ntile - 1) wave_moments(m, wm[c]);
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) cur[k] = nxt[k];
   }
+  __syncthreads();
+  if (threadIdx.x < G.n_cam * kMom) {
+    const int c = threadIdx.x / kMom, k = threadIdx.x % kMom;
+    double acc = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) acc += wm[c][w][k];
+    M[c][k] = acc;
+  }
+  __syncthreads();
+  fit_epilogue(G, P, p, M, s, S, partials);
+}
+
+// Variant "camera": one workgroup per (scan position, camera length) --
+// n_cam times more, shorter work items, for scans too small to fill the
+// device one workgroup per position.  Each pays the per-workgroup set-up
+// (slopes, scan position, camera constants) without a second tile to
+// overlap it with, so on large scans it is slower than "scan".  Writes the camera's FP64 moments; stem_fit_epilogue
+// finishes each scan position with the same arithmetic as the scan variant.
+__global__ __launch_bounds__(kThreads) void stem_fit_cam_kernel(const tg_stem_geometry G,
+                                                                const double* __restrict__ P,
+                                                                const float* __restrict__ T,
+                                                                double* __restrict__ cam_mom) {
+  __shared__ float th[kMaxSide];
+  __shared__ double wm[kWaves][kMom];
+  const int64_t S = (int64_t)G.scan_h * G.scan_w;
+  const int64_t J = (int64_t)G.n_side * G.n_side;
+  fill_slopes(G, th);
+  __syncthreads();
+  const int64_t npair = J >> 1;
+  const int64_t ntile = (npair + kTile - 1) / kTile;
+  const f4* __restrict__ T4 = (const f4*)T;
+  const int64_t s = blockIdx.x;
+  const int c = blockIdx.y;
+  f4 cur[kPerLane], nxt[kPerLane];
+  load_tile(T4, S, s, npair, ntile, (int64_t)c * ntile, cur);
+  const ScanPos p = scan_position(G, P, s);
+  const CamConst K = cam_const(G, P, p, c);
+  float m[kMom];
+#pragma unroll
+  for (int k = 0; k < kMom; ++k) m[k] = 0.f;
+  for (int64_t tile = 0; tile < ntile; ++tile) {
+    if (tile + 1 < ntile) load_tile(T4, S, s, npair, ntile, (int64_t)c * ntile + tile + 1, nxt);
+    accumulate_tile(G, th, K, cur, tile, npair, m);
+#pragma unroll
+    for (int k = 0; k < kPerLane; ++k) cur[k] = nxt[k];
+  }
+  wave_moments(m, wm);
+  __syncthreads();
+  if (threadIdx.x < kMom) {
+    double acc = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) acc += wm[w][threadIdx.x];
+    cam_mom[(s * G.n_cam + c) * kMom + threadIdx.x] = acc;
+  }
+}
+
+__global__ __launch_bounds__(64) void stem_fit_epilogue(const tg_stem_geometry G, const double* __restrict__ P,
+                                                        const double* __restrict__ cam_mom,
+                                                        double* __restrict__ partials) {
+  __shared__ double M[TG_STEM_MAX_CAM][kMom];
+  const int64_t S = (int64_t)G.scan_h * G.scan_w;
+  const int64_t s = blockIdx.x;
+  for (int i = threadIdx.x; i < G.n_cam * kMom; i += 64) M[i / kMom][i % kMom] = cam_mom[s * G.n_cam * kMom + i];
+  __syncthreads();
+  const ScanPos p = scan_position(G, P, s);
+  fit_epilogue(G, P, p, M, s, S, partials);
 }
 
 // Fixed-order FP64 row sums of the [np, rows] workspace (one block per row,
@@ -387,18 +490,42 @@ extern "C" int tg_stem_forward(const tg_stem_geometry* g, const double* params, 
   return TG_OK;
 }
 
-extern "C" int tg_stem_fit(const tg_stem_geometry* g, const double* params, const float* targets, double* partials,
+extern "C" int64_t tg_stem_fit_workspace(const tg_stem_geometry* g) {
+  if (g == nullptr || g->scan_h <= 0 || g->scan_w <= 0 || g->n_cam < 1 || g->n_cam > TG_STEM_MAX_CAM) return 0;
+  const int64_t S = (int64_t)g->scan_h * g->scan_w;
+  return (int64_t)sizeof(double) * S * ((18 + g->n_cam + 1) + (int64_t)g->n_cam * kMom);
+}
+
+extern "C" int tg_stem_fit(const tg_stem_geometry* g, const double* params, const float* targets, double* workspace,
                            double* out, void* stream) {
   tg::clear_error();
   int st = validate(g, params);
   if (st != TG_OK) return st;
-  TG_REQUIRE(targets != nullptr && partials != nullptr && out != nullptr, "tg_stem_fit: NULL buffer");
+  TG_REQUIRE(targets != nullptr && workspace != nullptr && out != nullptr, "tg_stem_fit: NULL buffer");
   TG_REQUIRE(((uintptr_t)targets & 15) == 0, "tg_stem_fit: targets must be 16-byte aligned");
   const int64_t S = (int64_t)g->scan_h * g->scan_w;
   const int np = 18 + g->n_cam + 1;
-  hipLaunchKernelGGL(stem_fit_kernel, dim3((unsigned)S), dim3(kThreads), 0, (hipStream_t)stream, *g, params, targets,
-                     partials);
-  TG_LAUNCH_CHECK("stem_fit_kernel launch");
+  double* partials = workspace;                   // [np, S]
+  double* cam_mom = workspace + (int64_t)np * S;  // [S, n_cam, 7]
+  // launch variant: the autotuner's choice, else one workgroup per scan
+  // position when that gives every CU several workgroups, and one per (scan
+  // position, camera length) for small scans
+  const int64_t key[4] = {S, (int64_t)g->n_side * g->n_side, g->n_cam, 0};
+  int32_t variant = S >= 4LL * tg::cu_count() ? TG_STEM_FIT_SCAN : TG_STEM_FIT_CAMERA;
+  int32_t tuned = 0;
+  if (tg::tune_lookup(TG_TUNE_STEM_FIT, key, tuned)) variant = tuned;
+  if (variant == TG_STEM_FIT_SCAN) {
+    hipLaunchKernelGGL(stem_fit_kernel, dim3((unsigned)S), dim3(kThreads), 0, (hipStream_t)stream, *g, params,
+                       targets, partials);
+    TG_LAUNCH_CHECK("stem_fit_kernel launch");
+  } else {
+    hipLaunchKernelGGL(stem_fit_cam_kernel, dim3((unsigned)S, (unsigned)g->n_cam), dim3(kThreads), 0,
+                       (hipStream_t)stream, *g, params, targets, cam_mom);
+    TG_LAUNCH_CHECK("stem_fit_cam_kernel launch");
+    hipLaunchKernelGGL(stem_fit_epilogue, dim3((unsigned)S), dim3(64), 0, (hipStream_t)stream, *g, params, cam_mom,
+                       partials);
+    TG_LAUNCH_CHECK("stem_fit_epilogue launch");
+  }
   hipLaunchKernelGGL(row_sum_kernel, dim3((unsigned)np), dim3(kSumThreads), 0, (hipStream_t)stream, partials, S, out);
   TG_LAUNCH_CHECK("row_sum_kernel launch");
   return TG_OK;

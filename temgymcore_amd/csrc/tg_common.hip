@@ -67,13 +67,13 @@ extern "C" {
 
 const char* tg_last_error(void) { return g_err; }
 
-int32_t tg_abi_version(void) { return 3; }
+int32_t tg_abi_version(void) { return 4; }
 
 int32_t tg_device_cu_count(void) { return tg::cu_count(); }
 
 int tg_tune_set(int32_t knob, const int64_t* key, int32_t value) {
   tg::clear_error();
-  TG_REQUIRE(knob >= TG_TUNE_BEAMLET_SPLITS && knob <= TG_TUNE_FIT_SLICE, "tg_tune_set: unknown knob %d", knob);
+  TG_REQUIRE(knob >= TG_TUNE_BEAMLET_SPLITS && knob <= TG_TUNE_STEM_FIT, "tg_tune_set: unknown knob %d", knob);
   TG_REQUIRE(key != nullptr, "tg_tune_set: NULL key");
   TG_REQUIRE(value > 0, "tg_tune_set: value must be positive");
   std::lock_guard<std::mutex> lk(g_tune_mu);

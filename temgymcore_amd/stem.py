@@ -47,6 +47,7 @@ class StemGeometry(ctypes.Structure):
 _P = ctypes.c_void_p
 _lib.register_signatures({
     "tg_stem_forward": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, _P, _P, _P, _P]),
+    "tg_stem_fit_workspace": (ctypes.c_int64, [ctypes.POINTER(StemGeometry)]),
     "tg_stem_fit": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, _P, _P, _P]),
 })
 
@@ -255,8 +256,10 @@ class STEMFitWorkspace:
         self.model = model
         self.geometry = model.geometry()
         self.np = 18 + model.n_cam + 1
-        # [np, S]: per-scan-position contributions, one row per output entry
-        self.partials = torch.empty((self.np, model.n_scan), dtype=torch.float64, device=device)
+        n = int(_lib.load().tg_stem_fit_workspace(ctypes.byref(self.geometry))) // 8
+        self.workspace = torch.empty(n, dtype=torch.float64, device=device)
+        # its head [np, S]: per-scan-position contributions, one row per output entry
+        self.partials = self.workspace[: self.np * model.n_scan].view(self.np, model.n_scan)
         self.out = torch.empty(self.np, dtype=torch.float64, device=device)
 
     def run(self, params: torch.Tensor, targets: torch.Tensor) -> torch.Tensor:
@@ -268,7 +271,7 @@ class STEMFitWorkspace:
                 or not targets.is_contiguous():
             raise ValueError(f"targets must be a contiguous device float32 tensor of shape {shape}")
         st = _lib.load().tg_stem_fit(ctypes.byref(self.geometry), params.data_ptr(), targets.data_ptr(),
-                                     self.partials.data_ptr(), self.out.data_ptr(),
+                                     self.workspace.data_ptr(), self.out.data_ptr(),
                                      _lib.stream_ptr(targets.device))
         _lib.check(st, "tg_stem_fit")
         return self.out

@@ -168,3 +168,24 @@ def test_autograd_function_and_graph_replay(device):
         torch.cuda.synchronize()
         torch.testing.assert_close(ws.out, eager, rtol=0, atol=0)
         _ = (targets[0, 0, 0, 0] * 1.0).item()  # eager work between replays
+
+
+def test_fit_launch_variants_are_bitwise_equal(device):
+    """The per-scan and per-(scan, camera) launch variants run the same
+    arithmetic in the same order."""
+    from temgymcore_amd import autotune
+
+    model, _ = stem.reference_workload(n_cam=3, scan_shape=(12, 10), n_side=32)
+    params = stem.params_tensor(model, device)
+    targets = stem.stem_forward(stem.perturbed(model))["coords"]
+    ws = stem.STEMFitWorkspace(model, device)
+    key = (model.n_scan, model.n_rays, model.n_cam)
+    outs = []
+    try:
+        for variant in (autotune.STEM_FIT_SCAN, autotune.STEM_FIT_CAMERA):
+            autotune.set_choice(autotune.STEM_FIT, key, variant)
+            outs.append((ws.run(params, targets).clone(), ws.partials.clone()))
+    finally:
+        autotune.clear()
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert torch.equal(outs[0][1], outs[1][1])

@@ -5,6 +5,7 @@ The compiled register programs are executed here by a small numpy
 interpreter that applies one IEEE operation per instruction, exactly as the
 kernel does; its results must be bit-identical to the oracle's f32 run."""
 import ctypes
+import dataclasses
 import os
 import re
 
@@ -18,7 +19,7 @@ from temgymcore_amd.components import (
     Biprism, Deflector, DescanError, Descanner, Detector, Lens, Plane, Rotator, Scanner,
 )
 from temgymcore_amd.ray import Ray
-from temgymcore_amd.run import _trace_model, run_iter, run_to_end, solve_model
+from temgymcore_amd.run import _split, _trace_model, run_iter, run_to_end, solve_model
 from temgymcore_amd.propagator import FreeSpaceParaxial
 from temgymcore_amd.transfer import custom_jacobian_matrix, jacobian, ray_matrix
 
@@ -355,3 +356,42 @@ def test_ray_matrix_through_biprism_vs_oracle_differences():
             ref[:, j] = (outs[0] - outs[1]) / 2e-10
         np.testing.assert_allclose(J, ref, rtol=1e-6, atol=1e-9)
         assert np.sign(J[2, 4]) == np.sign(x0)
+
+
+def test_traced_jacobian_with_many_outputs_is_split_by_outputs():
+    """A forward-mode trace of a Biprism model stores more values (outputs and
+    their tangents) than one program holds; the extra outputs get programs of
+    their own.  Interpreted, every column equals the host matrix's."""
+    model = [Lens(z=2e-4, focal_length=5e-4), Biprism(z=4e-4, def_x=1e-5),
+             Deflector(z=6e-4, def_x=1e-6, def_y=-2e-6), Detector(z=1e-3, pixel_size=(3e-8, 3e-8), shape=(8, 8))]
+    rng = np.random.default_rng(32)
+    n = 6
+    cols = {k: torch.from_numpy(rng.uniform(-2e-7, 2e-7, n)) for k in ("x", "y")}
+    cols.update({k: torch.from_numpy(rng.uniform(-1e-5, 1e-5, n)) for k in ("dx", "dy")})
+    ray = Ray(x=cols["x"], y=cols["y"], dx=cols["dx"], dy=cols["dy"], z=torch.zeros(n, dtype=torch.float64),
+              pathlength=torch.zeros(n, dtype=torch.float64), _one=torch.ones(n, dtype=torch.float64))
+    fields = ("x", "y", "dx", "dy", "_one")
+    J = np.zeros((n, 5, 5))
+    n_split = 0
+    for j, fin in enumerate(fields):
+        tr = _program.Trace(torch.device("cpu"))
+        seeded = dataclasses.replace(ray, **{fin: _program.FwdDual(getattr(ray, fin), 1.0)})
+        val, tan = _split(_trace_model(tr.symbolize(seeded), model, FreeSpaceParaxial()), ray)
+        syms = [getattr(r, f.name) for r in (val, tan) for f in dataclasses.fields(r)]
+        vids = list(dict.fromkeys(v.vid for v in syms if isinstance(v, _program.Sym) and v.vid not in tr.value_slot))
+        prefixes, last, prefix_comps = _program._compile_segments(tr, vids, [], (n,))
+        n_split = max(n_split, len(prefixes))
+        got = {}
+        for comp, outs in prefix_comps:
+            res = interpret(tr, comp)
+            for v, t in outs.items():
+                t.copy_(torch.from_numpy(res[v]))
+            got.update(res)
+        got.update(interpret(tr, last))
+        for i, fout in enumerate(fields):
+            t = getattr(tan, fout)
+            J[:, i, j] = got[t.vid] if isinstance(t, _program.Sym) else t
+    assert n_split >= 1
+    for b in range(n):
+        r = Ray(float(cols["x"][b]), float(cols["y"][b]), float(cols["dx"][b]), float(cols["dy"][b]), 0.0, 0.0)
+        np.testing.assert_allclose(J[b], np.asarray(ray_matrix(model, r)), rtol=1e-12, atol=1e-15)
