@@ -31,7 +31,7 @@ extern "C" {
 const char* tg_last_error(void);
 
 /* ABI version; bumped whenever a struct layout below changes. */
-int32_t tg_abi_version(void);  /* currently 4 */
+int32_t tg_abi_version(void);  /* currently 5 */
 
 /* ======================================================================== *
  * Ray branch: per-ray register programs
@@ -47,9 +47,13 @@ int32_t tg_abi_version(void);  /* currently 4 */
  * dtype, as weakly typed Python scalars are in the reference).
  * ======================================================================== */
 
-#define TG_MAX_INSTR 96
-#define TG_MAX_REG 16
-#define TG_MAX_SLOT 10
+#define TG_MAX_INSTR 112
+/* Programs whose registers all lie below 16 run on the kernel with a
+ * 16-register file in VGPRs (4 f32 / 2 f64 rays per lane); wider ones (e.g.
+ * the forward-mode tangents of nonlinear elements) on a kernel that keeps
+ * TG_MAX_REG registers per ray in scratch memory (one ray per lane). */
+#define TG_MAX_REG 64
+#define TG_MAX_SLOT 32
 #define TG_MAX_DIM 4
 
 enum tg_opcode {

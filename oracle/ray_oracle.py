@@ -11,6 +11,7 @@ A model is a list of (kind, params) tuples:
   ("plane", z) ("lens", z, f) ("detector", z) ("scanner", z, spx, spy, stx, sty)
   ("descanner", z, spx, spy, stx, sty, err12) ("deflector", z, defx, defy)
   ("rotator", z, angle_deg) ("biprism", z, def_x)
+  ("kriv", z, f, coeffs) -- AberratedLensKrivanek, coeffs a dict of KRIVANEK_FIELDS
 
 Pinning (tests/test_oracle_ray.py):
   * README golden: Ray(0.1, 0.2, 0.3, 0.4, 0, 0) through Lens(0.5, f=1) and
@@ -88,6 +89,99 @@ def biprism(r, def_x):
     return dict(r, dx=dx + def_x * r["one"] * np.sign(x), pathlength=r["pathlength"] + dx * x + dy * y)
 
 
+KRIVANEK_FIELDS = ("C10", "C12", "phi12", "C21", "phi21", "C23", "phi23", "C30", "C32", "phi32", "C34", "phi34",
+                   "C41", "phi41", "C43", "phi43", "C45", "phi45", "C50", "C52", "phi52", "C54", "phi54",
+                   "C56", "phi56")
+
+
+def _kc(c, m, ph, ph0):
+    return np.cos(m * (ph - ph0))
+
+
+def _ks(c, m, ph, ph0):
+    return np.sin(m * (ph - ph0))
+
+
+def krivanek_brackets(phi, c):
+    """B2..B6 (reference aberrations.py:42-48); c: dict of KRIVANEK_FIELDS (missing = 0)."""
+    g = lambda k: c.get(k, 0.0)  # noqa: E731
+    B2 = g("C10") + g("C12") * _kc(c, 2, phi, g("phi12"))
+    B3 = g("C21") * _kc(c, 1, phi, g("phi21")) + g("C23") * _kc(c, 3, phi, g("phi23"))
+    B4 = g("C30") + g("C32") * _kc(c, 2, phi, g("phi32")) + g("C34") * _kc(c, 4, phi, g("phi34"))
+    B5 = (g("C41") * _kc(c, 1, phi, g("phi41")) + g("C43") * _kc(c, 3, phi, g("phi43"))
+          + g("C45") * _kc(c, 5, phi, g("phi45")))
+    B6 = (g("C50") + g("C52") * _kc(c, 2, phi, g("phi52")) + g("C54") * _kc(c, 4, phi, g("phi54"))
+          + g("C56") * _kc(c, 6, phi, g("phi56")))
+    return B2, B3, B4, B5, B6
+
+
+def krivanek_W(alpha, phi, c):
+    """W_krivanek (reference aberrations.py:51-61)."""
+    B2, B3, B4, B5, B6 = krivanek_brackets(phi, c)
+    a = alpha
+    a2 = a * a
+    a3 = a2 * a
+    a4 = a2 * a2
+    a6 = a3 * a3
+    return 0.5 * a2 * B2 + (a3 / 3.0) * B3 + 0.25 * a4 * B4 + 0.2 * a4 * a * B5 + (a6 / 6.0) * B6
+
+
+def krivanek_grad(ax, ay, c):
+    """grad_W_krivanek (reference aberrations.py:64-108): dW/dalpha and
+    dW/dphi of the polar series, mapped to (ax, ay) with alpha -> 1e-30 at 0."""
+    g = lambda k: c.get(k, 0.0)  # noqa: E731
+    a = np.hypot(ax, ay)
+    phi = np.arctan2(ay, ax)
+    B2, B3, B4, B5, B6 = krivanek_brackets(phi, c)
+    a2 = a * a
+    a3 = a2 * a
+    a4 = a2 * a2
+    a5 = a4 * a
+    a6 = a3 * a3
+    dW_da = a * B2 + a2 * B3 + a3 * B4 + a4 * B5 + a5 * B6
+    dW_dphi = (0.5 * a2) * (-2.0 * g("C12") * _ks(c, 2, phi, g("phi12")))
+    dW_dphi = dW_dphi + (a3 / 3.0) * (-1.0 * g("C21") * _ks(c, 1, phi, g("phi21"))
+                                      - 3.0 * g("C23") * _ks(c, 3, phi, g("phi23")))
+    dW_dphi = dW_dphi + (0.25 * a4) * (-2.0 * g("C32") * _ks(c, 2, phi, g("phi32"))
+                                       - 4.0 * g("C34") * _ks(c, 4, phi, g("phi34")))
+    dW_dphi = dW_dphi + (0.2 * a4 * a) * (-1.0 * g("C41") * _ks(c, 1, phi, g("phi41"))
+                                          - 3.0 * g("C43") * _ks(c, 3, phi, g("phi43"))
+                                          - 5.0 * g("C45") * _ks(c, 5, phi, g("phi45")))
+    dW_dphi = dW_dphi + (a6 / 6.0) * (-2.0 * g("C52") * _ks(c, 2, phi, g("phi52"))
+                                      - 4.0 * g("C54") * _ks(c, 4, phi, g("phi54"))
+                                      - 6.0 * g("C56") * _ks(c, 6, phi, g("phi56")))
+    a_safe = np.where(a == 0, 1e-30, a)
+    inv_a = 1.0 / a_safe
+    inv_a2 = inv_a * inv_a
+    dWx = dW_da * (ax * inv_a) + dW_dphi * (-ay * inv_a2)
+    dWy = dW_da * (ay * inv_a) + dW_dphi * (ax * inv_a2)
+    return dWx, dWy
+
+
+def aberrated_lens(r, f, c):
+    """AberratedLensKrivanek (reference components.py:192-214)."""
+    x, y = r["x"], r["y"]
+    idx = -x / f + r["dx"]
+    idy = -y / f + r["dy"]
+    alpha = np.hypot(idx, idy)
+    phi = np.arctan2(idy, idx)
+    dWx, dWy = krivanek_grad(idx, idy, c)
+    return dict(r, dx=idx + (-dWx / f), dy=idy + (-dWy / f),
+                pathlength=r["pathlength"] - (x * x + y * y) / (2 * f) + krivanek_W(alpha, phi, c) / f,
+                one=r["one"] * 1.0)
+
+
+def fibonacci_spiral(nb_samples, radius, alpha=2):
+    """Fibonacci-spiral disc points (reference utils.py:297-325)."""
+    ga = np.pi * (3.0 - np.sqrt(5.0))
+    boundary = np.round(alpha * np.sqrt(nb_samples))
+    i = np.arange(nb_samples)
+    rr = np.where(i > nb_samples - (boundary + 1), radius,
+                  radius * np.sqrt((i + 0.5) / (nb_samples - 0.5 * (boundary + 1))))
+    rr[0] = 0.0
+    return rr * np.cos(i * ga), rr * np.sin(i * ga)
+
+
 def apply(r, comp):
     kind, *p = comp
     if kind in ("plane", "detector"):
@@ -102,6 +196,8 @@ def apply(r, comp):
         return deflector(r, *p[1:])
     if kind == "rotator":
         return rotator(r, p[1])
+    if kind == "kriv":
+        return aberrated_lens(r, p[1], p[2])
     if kind == "biprism":
         return biprism(r, p[1])
     raise ValueError(kind)

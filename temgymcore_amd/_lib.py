@@ -15,11 +15,11 @@ import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtemgym_hip.so")
-ABI_VERSION = 4
+ABI_VERSION = 5
 
-TG_MAX_INSTR = 96
-TG_MAX_REG = 16
-TG_MAX_SLOT = 10
+TG_MAX_INSTR = 112
+TG_MAX_REG = 64
+TG_MAX_SLOT = 32
 TG_MAX_DIM = 4
 
 TG_F32 = 0

@@ -518,11 +518,14 @@ def _compile_segments(trace: Trace, out_vids: list, keep_vids: list, shape: tupl
     """Compile one shape group, splitting it into several programs when it
     exceeds one program's instruction, register, slot or output limits.
 
-    A prefix program stores every value that crosses its cut to a buffer of
-    the group's shape; those buffers become input slots of the rest, which is
-    compiled the same way.  Returns ([(Program, None, keepalive)] of the
-    prefixes, Compiled of the last segment, [(Compiled, outputs)] of the
-    prefixes)."""
+    The group is cut after a prefix of its instructions.  The prefix program
+    stores the final outputs it has computed and every value the rest still
+    reads; those buffers become input slots of the rest, which is compiled
+    the same way.  A cut is admissible when the rest's inputs (buffers and
+    the original inputs it still reads) fit in one program's slots; the
+    latest admissible cut whose prefix compiles is taken.  Returns
+    ([(Program, None, keepalive)] of the prefixes, Compiled of the last
+    segment, [(Compiled, outputs)] of the prefixes)."""
     limits = (_TooLong, _TooWide, _TooManyRegs)
     prefixes, prefix_comps = [], []
     out_vids = list(dict.fromkeys(out_vids))
@@ -545,22 +548,29 @@ def _compile_segments(trace: Trace, out_vids: list, keep_vids: list, shape: tupl
         chosen = None
         for c in range(min(len(kept) - 1, _lib.TG_MAX_INSTR), 0, -1):
             defined = {ins.dst for ins in kept[:c]}
-            used_after = {v for ins in kept[c:] for v in (ins.a, ins.b)} | live_out
-            cross = sorted(defined & used_after)
-            if not cross or len(cross) > _lib.TG_MAX_SLOT:
+            reads_after = {v for ins in kept[c:] for v in (ins.a, ins.b)} | set(keep_vids)
+            cross = defined & reads_after
+            stores = sorted(cross | (defined & set(out_vids)))
+            if not stores or len(stores) > _lib.TG_MAX_SLOT:
+                continue
+            still = reads_after | (live_out - defined)
+            slot_in = {v for v in still if v in trace.value_slot}
+            if len(slot_in | cross) > _lib.TG_MAX_SLOT:
                 continue
             try:
-                chosen = (cross, compile_group(trace, cross, [], shape))
+                chosen = (stores, compile_group(trace, stores, [], shape))
             except limits:
                 continue
             break
         if chosen is None:
             raise NotImplementedError("ray program cannot be split into programs that fit the kernel's limits")
-        cross, comp = chosen
+        stores, comp = chosen
         prog, outs, keepalive = prepare(trace, comp, None)
         prefixes.append((prog, None, keepalive + list(outs.values())))
         prefix_comps.append((comp, outs))
         _bind_slots(trace, outs)
+        # final outputs stored by the prefix need no later program
+        out_vids = [v for v in out_vids if v not in outs]
 
 
 def _bind_slots(trace: Trace, outs: dict):

@@ -124,6 +124,38 @@ class Lens(Component):
 
 
 @dataclasses.dataclass(frozen=True)
+class AberratedLensKrivanek(Lens):
+    """Thin lens with Krivanek aberrations (reference components.py:177-215).
+
+    The paraxial slopes a = (-x/f + dx, -y/f + dy) are deflected by -grad W(a)/f
+    and the pathlength gains W(a)/f, with W the Krivanek series of ``coeffs``
+    (a KrivanekCoeffs, or a dict of its fields).  W and its gradient are
+    evaluated as polynomials in the slopes (aberrations.krivanek_poly), so
+    the per-ray program needs no transcendental functions."""
+
+    coeffs: object = None
+
+    def __post_init__(self):
+        from .aberrations import as_coeffs
+
+        object.__setattr__(self, "coeffs", as_coeffs(self.coeffs if self.coeffs is not None else {}))
+
+    def __call__(self, ray: Ray):
+        from .aberrations import krivanek_poly
+
+        f = self.focal_length
+        x, y, dx, dy = ray.x, ray.y, ray.dx, ray.dy
+        ideal_dx = -x / f + dx
+        ideal_dy = -y / f + dy
+        W, dWx, dWy = krivanek_poly(ideal_dx, ideal_dy, self.coeffs)
+        aber_dx = ideal_dx + (-dWx / f)
+        aber_dy = ideal_dy + (-dWy / f)
+        pathlength = ray.pathlength - (x**2 + y**2) / (2 * f) + W / f
+        one = ray._one * 1.0
+        return Ray(x=x, y=y, dx=aber_dx, dy=aber_dy, _one=one, pathlength=pathlength, z=ray.z)
+
+
+@dataclasses.dataclass(frozen=True)
 class ScanGrid(Component, Grid):
     """Scan grid: pixel<->metre mapping at plane z; a no-op on rays (reference components.py:218-249)."""
 

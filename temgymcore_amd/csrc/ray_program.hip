@@ -22,11 +22,15 @@
 // Arithmetic is exactly one IEEE operation per instruction, built with
 // -ffp-contract=off, so results are bit-identical to the same operation
 // sequence evaluated elementwise on the CPU.
+#include <algorithm>
+
 #include "tg_common.hpp"
 
 namespace {
 
 constexpr int kThreads = 256;
+// the program and binning spec travel by value in the kernel-argument segment
+static_assert(sizeof(tg_program) + sizeof(tg_bin_spec) <= 4096, "ray program exceeds the kernel-argument limit");
 constexpr int64_t kLdsBins = 32768;
 constexpr int64_t kMaxRaysPerGroup = 65535;
 
@@ -40,16 +44,43 @@ __device__ __forceinline__ double imm_of<double>(const tg_instr& in) { return in
 // The register file of each ray is one 16-wide vector value.  Indexing it
 // with a uniform (scalar) register number lowers to the VGPR-indexing mode
 // (s_set_gpr_idx_on / v_movrel*): no scratch memory and no selection trees.
+// The host only dispatches programs whose registers are all below 16 here.
 template <typename T>
-struct RegVec;
+struct Vec16;
 template <>
-struct RegVec<float> {
+struct Vec16<float> {
   typedef float type __attribute__((ext_vector_type(16)));
 };
 template <>
-struct RegVec<double> {
+struct Vec16<double> {
   typedef double type __attribute__((ext_vector_type(16)));
 };
+
+template <typename T, int NREG>
+struct RegFile;
+template <typename T>
+struct RegFile<T, 16> {
+  typename Vec16<T>::type v;
+  __device__ __forceinline__ void clear() { v = (typename Vec16<T>::type)(T(0)); }
+  __device__ __forceinline__ T get(int i) const { return v[i]; }
+  __device__ __forceinline__ void set(int i, T x) { v[i] = x; }
+};
+// Wide programs (more than 16 live values, e.g. forward-mode tangents of
+// nonlinear elements) keep their TG_MAX_REG registers in a per-lane array in
+// scratch memory, indexed in bounds by masking.  (A VGPR file wider than 16
+// is not safe to build from vector parts: the compiler if-converts the part
+// selection and performs the indexed write of every part, which for an index
+// of another part lands on unrelated VGPRs.)  Registers are not cleared:
+// programs only use values they have loaded or computed, and the unused
+// operand of a unary instruction is read but never used.
+template <typename T>
+struct RegFile<T, TG_MAX_REG> {
+  T v[TG_MAX_REG];
+  __device__ __forceinline__ void clear() {}
+  __device__ __forceinline__ T get(int i) const { return v[i & (TG_MAX_REG - 1)]; }
+  __device__ __forceinline__ void set(int i, T x) { v[i & (TG_MAX_REG - 1)] = x; }
+};
+static_assert((TG_MAX_REG & (TG_MAX_REG - 1)) == 0, "TG_MAX_REG must be a power of two");
 
 template <typename T>
 __device__ __forceinline__ T sign_of(T a) {
@@ -102,7 +133,7 @@ __device__ __forceinline__ void global_deposit(const tg_bin_spec& B, int64_t p, 
 // BIN: 0 = no binning, 1 = LDS-privatised integer histogram, 2 = global atomics.
 // NT: workgroup size (LDS-histogram launches use 1024 so that 16 wavefronts
 // share one private histogram and fewer flushes hit the same global bins).
-template <typename T, int R, bool CONTIG, int BIN, int NT>
+template <typename T, int R, bool CONTIG, int BIN, int NT, int NREG>
 __global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, const tg_bin_spec B) {
   constexpr int kThreads = NT;
   extern __shared__ int32_t lds_hist[];
@@ -115,10 +146,9 @@ __global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, con
   }
   const int64_t chunk = (int64_t)kThreads * R;
   for (int64_t base = (int64_t)blockIdx.x * chunk; base < n; base += (int64_t)gridDim.x * chunk) {
-    typedef typename RegVec<T>::type VT;
-    VT r[R];
+    RegFile<T, NREG> r[R];
 #pragma unroll
-    for (int u = 0; u < R; ++u) r[u] = (VT)(T(0));
+    for (int u = 0; u < R; ++u) r[u].clear();
     int64_t idx[R];
     bool ok[R];
 #pragma unroll
@@ -140,7 +170,7 @@ __global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, con
       }
       const int dst = P.slot[s].reg;
 #pragma unroll
-      for (int u = 0; u < R; ++u) r[u][dst] = v[u];
+      for (int u = 0; u < R; ++u) r[u].set(dst, v[u]);
     }
     for (int i = 0; i < P.n_instr; ++i) {
       const tg_instr in = P.instr[i];
@@ -148,8 +178,8 @@ __global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, con
       T a[R], b[R], d[R];
 #pragma unroll
       for (int u = 0; u < R; ++u) {
-        a[u] = r[u][ia];
-        b[u] = r[u][ib];
+        a[u] = r[u].get(ia);
+        b[u] = r[u].get(ib);
       }
       const bool f32op = sizeof(T) == 8 && (in.op & TG_OP_F32);
       const T im = f32op ? (T)in.immf : imm_of<T>(in);
@@ -216,13 +246,13 @@ __global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, con
         for (int u = 0; u < R; ++u) d[u] = (T)(float)d[u];
       }
 #pragma unroll
-      for (int u = 0; u < R; ++u) r[u][id] = d[u];
+      for (int u = 0; u < R; ++u) r[u].set(id, d[u]);
     }
     for (int k = 0; k < P.n_out; ++k) {
       T v[R];
       const int src = P.out_reg[k];
 #pragma unroll
-      for (int u = 0; u < R; ++u) v[u] = r[u][src];
+      for (int u = 0; u < R; ++u) v[u] = r[u].get(src);
       if (sizeof(T) == 8 && P.out_dtype[k] == TG_F32) {
         float* __restrict__ o = (float*)P.out[k];
 #pragma unroll
@@ -240,7 +270,7 @@ __global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, con
       if (B.x_reg >= 0) {
         const int src = B.x_reg;
 #pragma unroll
-        for (int u = 0; u < R; ++u) xv[u] = r[u][src];
+        for (int u = 0; u < R; ++u) xv[u] = r[u].get(src);
       } else {
 #pragma unroll
         for (int u = 0; u < R; ++u) xv[u] = (T)B.x_uniform;
@@ -248,7 +278,7 @@ __global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, con
       if (B.y_reg >= 0) {
         const int src = B.y_reg;
 #pragma unroll
-        for (int u = 0; u < R; ++u) yv[u] = r[u][src];
+        for (int u = 0; u < R; ++u) yv[u] = r[u].get(src);
       } else {
 #pragma unroll
         for (int u = 0; u < R; ++u) yv[u] = (T)B.y_uniform;
@@ -256,7 +286,7 @@ __global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, con
       if (B.w_reg >= 0) {
         const int src = B.w_reg;
 #pragma unroll
-        for (int u = 0; u < R; ++u) wv[u] = r[u][src];
+        for (int u = 0; u < R; ++u) wv[u] = r[u].get(src);
       } else {
 #pragma unroll
         for (int u = 0; u < R; ++u) wv[u] = T(1);
@@ -293,7 +323,7 @@ __global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, con
   }
 }
 
-template <typename T, int R, bool CONTIG, int BIN>
+template <typename T, int R, bool CONTIG, int BIN, int NREG>
 int launch(const tg_program& P, const tg_bin_spec& B, hipStream_t stream) {
   constexpr int kThreads = (BIN == 1) ? 1024 : 256;
   const int64_t chunk = (int64_t)kThreads * R;
@@ -318,22 +348,39 @@ int launch(const tg_program& P, const tg_bin_spec& B, hipStream_t stream) {
     if (grid < min_grid) grid = min_grid;
   }
   const size_t lds = (BIN == 1) ? (size_t)((B.height * B.width + 1) / 2) * sizeof(int32_t) : 0;
-  hipLaunchKernelGGL((ray_program_kernel<T, R, CONTIG, BIN, kThreads>), dim3((unsigned)grid), dim3(kThreads), lds,
-                     stream, P, B);
+  hipLaunchKernelGGL((ray_program_kernel<T, R, CONTIG, BIN, kThreads, NREG>), dim3((unsigned)grid), dim3(kThreads),
+                     lds, stream, P, B);
   TG_LAUNCH_CHECK("ray_program_kernel launch");
   return TG_OK;
 }
 
-template <typename T, int R>
+template <typename T, int R, int NREG>
 int dispatch(const tg_program& P, const tg_bin_spec& B, bool contig, int bin, hipStream_t s) {
+  // wide programs bin with global atomics (one kernel fewer per file width)
+  if (NREG > 16 && bin == 1) bin = 2;
   if (contig) {
-    if (bin == 0) return launch<T, R, true, 0>(P, B, s);
-    if (bin == 1) return launch<T, R, true, 1>(P, B, s);
-    return launch<T, R, true, 2>(P, B, s);
+    if (bin == 0) return launch<T, R, true, 0, NREG>(P, B, s);
+    if constexpr (NREG == 16)
+      if (bin == 1) return launch<T, R, true, 1, NREG>(P, B, s);
+    return launch<T, R, true, 2, NREG>(P, B, s);
   }
-  if (bin == 0) return launch<T, R, false, 0>(P, B, s);
-  if (bin == 1) return launch<T, R, false, 1>(P, B, s);
-  return launch<T, R, false, 2>(P, B, s);
+  if (bin == 0) return launch<T, R, false, 0, NREG>(P, B, s);
+  if constexpr (NREG == 16)
+    if (bin == 1) return launch<T, R, false, 1, NREG>(P, B, s);
+  return launch<T, R, false, 2, NREG>(P, B, s);
+}
+
+// Highest register the program (and its binning) touches.
+int max_register(const tg_program& P, const tg_bin_spec* B) {
+  int m = 0;
+  for (int i = 0; i < P.n_instr; ++i) {
+    const tg_instr& in = P.instr[i];
+    m = std::max(m, (int)std::max(in.dst, std::max(in.a, in.b)));
+  }
+  for (int k = 0; k < P.n_slot; ++k) m = std::max(m, (int)P.slot[k].reg);
+  for (int k = 0; k < P.n_out; ++k) m = std::max(m, (int)P.out_reg[k]);
+  if (B != nullptr) m = std::max(m, (int)std::max(B->x_reg, std::max(B->y_reg, B->w_reg)));
+  return m;
 }
 
 }  // namespace
@@ -392,6 +439,10 @@ extern "C" int tg_ray_program_run(const tg_program* prog, const tg_bin_spec* bin
   }
   if (P.n == 0) return TG_OK;
   hipStream_t s = (hipStream_t)stream;
-  if (P.dtype == TG_F32) return dispatch<float, 4>(P, B, contig, mode, s);
-  return dispatch<double, 2>(P, B, contig, mode, s);
+  if (max_register(P, mode != 0 ? &B : nullptr) < 16) {
+    if (P.dtype == TG_F32) return dispatch<float, 4, 16>(P, B, contig, mode, s);
+    return dispatch<double, 2, 16>(P, B, contig, mode, s);
+  }
+  if (P.dtype == TG_F32) return dispatch<float, 1, TG_MAX_REG>(P, B, contig, mode, s);
+  return dispatch<double, 1, TG_MAX_REG>(P, B, contig, mode, s);
 }

@@ -218,10 +218,15 @@ def _abcd_traced(rays: GaussianRay, model, device, n: int) -> torch.Tensor:
 def _abcd_per_beamlet(rays: GaussianRay, model, device, n: int):
     """(abcd tensor [m, 5, 5] f64, stride) of the model for each beamlet's z
     (or for each beamlet, when the model has ray-dependent elements)."""
-    from .components import Biprism
+    from .components import Deflector, Descanner, Detector, Lens, Plane, Rotator, ScanGrid, Scanner
+    from .source import Source
     from .transfer import ray_matrix
 
-    if any(isinstance(c, Biprism) for c in model):
+    # elements whose map is affine in (x, y, dx, dy, 1): one matrix serves every
+    # beamlet; any other element (Biprism, AberratedLensKrivanek, user
+    # components) gets each beamlet's own Jacobian
+    affine = (Deflector, Descanner, Detector, Lens, Plane, Rotator, ScanGrid, Scanner)
+    if not all(type(c) in affine or isinstance(c, Source) for c in model):
         if n == 0:
             return torch.zeros((1, 5, 5), dtype=torch.float64, device=device), 0
         return _abcd_traced(rays, model, device, n), 25

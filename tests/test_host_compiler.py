@@ -358,10 +358,12 @@ def test_ray_matrix_through_biprism_vs_oracle_differences():
         assert np.sign(J[2, 4]) == np.sign(x0)
 
 
-def test_traced_jacobian_with_many_outputs_is_split_by_outputs():
-    """A forward-mode trace of a Biprism model stores more values (outputs and
-    their tangents) than one program holds; the extra outputs get programs of
-    their own.  Interpreted, every column equals the host matrix's."""
+def test_traced_jacobian_with_many_outputs_is_split_by_outputs(monkeypatch):
+    """A forward-mode trace of a Biprism model stores 11-12 values (outputs
+    and their tangents); with a program limit of 10 outputs the extra outputs
+    get programs of their own.  Interpreted, every column equals the host
+    matrix's."""
+    monkeypatch.setattr(_lib, "TG_MAX_SLOT", 10)
     model = [Lens(z=2e-4, focal_length=5e-4), Biprism(z=4e-4, def_x=1e-5),
              Deflector(z=6e-4, def_x=1e-6, def_y=-2e-6), Detector(z=1e-3, pixel_size=(3e-8, 3e-8), shape=(8, 8))]
     rng = np.random.default_rng(32)

@@ -196,3 +196,114 @@ def test_planted_rounding_mistake_is_caught():
     good = (int(O.round_to_i32(py)), int(O.round_to_i32(px)))
     bad = (int(np.floor(py - 1e-9)), int(np.floor(px - 1e-9)))
     assert good == (0, 0) and bad != good
+
+
+# --------------------------------------------------------------------------
+# AberratedLensKrivanek (reference components.py:177-215, aberrations.py)
+# --------------------------------------------------------------------------
+def _golden_aberrated():
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "aberrated_probe_rays.json")) as fh:
+        return json.load(fh)
+
+
+def test_krivanek_notebook_golden_rays():
+    """The reference's printed run_to_end of examples/aberrated_probe.ipynb
+    (cell 9): spherical aberration C30 = 1e8 m at f = 0.25 mm.  The edge
+    rays' detector positions are pure aberration (x = -dW/dax)."""
+    g = _golden_aberrated()
+    st = g["setup"]
+    x, y = O.fibonacci_spiral(st["n_beams"], st["radius"], alpha=st["alpha"])
+    n, F1 = len(x), st["F1"]
+    r = O.make_ray(x, y, np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n), np.ones(n))
+    o = O.run_to_end(r, [("plane", 0.0), ("kriv", F1, F1, st["coeffs"]), ("detector", 2 * F1)])
+    idx = np.array(g["index"])
+    for k in ("x", "y", "dx", "dy"):
+        np.testing.assert_allclose(o[k][idx], g[k], rtol=6e-9, atol=0, err_msg=k)
+    np.testing.assert_allclose(o["pathlength"][idx], g["pathlength"], rtol=1e-8)
+
+
+def _kriv_random(rng, scale=1.0):
+    c = {}
+    for k in O.KRIVANEK_FIELDS:
+        c[k] = rng.uniform(-np.pi, np.pi) if k.startswith("phi") else scale * rng.uniform(-1, 1)
+    return c
+
+
+def test_krivanek_gradient_vs_central_differences():
+    """grad_W (polar derivatives mapped to slopes) against Richardson central
+    differences of W itself, every coefficient non-zero."""
+    rng = np.random.default_rng(41)
+    c = _kriv_random(rng)
+    ax, ay = rng.uniform(-0.6, 0.6, 50), rng.uniform(-0.6, 0.6, 50)
+
+    def W(u, v):
+        return O.krivanek_W(np.hypot(u, v), np.arctan2(v, u), c)
+
+    def cd(f, h):
+        return (f(h) - f(-h)) / (2 * h)
+
+    h = 1e-4
+    gx_fd = (4 * cd(lambda e: W(ax + e, ay), h / 2) - cd(lambda e: W(ax + e, ay), h)) / 3
+    gy_fd = (4 * cd(lambda e: W(ax, ay + e), h / 2) - cd(lambda e: W(ax, ay + e), h)) / 3
+    gx, gy = O.krivanek_grad(ax, ay, c)
+    np.testing.assert_allclose(gx, gx_fd, rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(gy, gy_fd, rtol=1e-8, atol=1e-10)
+
+
+def test_krivanek_closed_forms():
+    """Defocus C10: W = C10 a^2/2, grad = C10 (ax, ay); spherical C30:
+    W = C30 a^4/4, grad = C30 a^2 (ax, ay); two-fold astigmatism C12 at
+    phi12: W = C12/2 a^2 cos(2(phi - phi12))."""
+    ax, ay = np.array([0.3, -0.1, 0.0]), np.array([0.2, 0.4, 0.0])
+    a2 = ax * ax + ay * ay
+    gx, gy = O.krivanek_grad(ax, ay, {"C10": 2.0})
+    np.testing.assert_allclose(gx, 2.0 * ax, rtol=1e-14)
+    np.testing.assert_allclose(gy, 2.0 * ay, rtol=1e-14)
+    gx, gy = O.krivanek_grad(ax, ay, {"C30": 3.0})
+    np.testing.assert_allclose(gx, 3.0 * a2 * ax, rtol=1e-14)
+    np.testing.assert_allclose(gy, 3.0 * a2 * ay, rtol=1e-14)
+    a, ph = np.sqrt(a2), np.arctan2(ay, ax)
+    np.testing.assert_allclose(O.krivanek_W(a, ph, {"C30": 3.0}), 0.75 * a2 * a2, rtol=1e-14)
+    np.testing.assert_allclose(O.krivanek_W(a, ph, {"C12": 2.0, "phi12": 0.4}),
+                               a2 * np.cos(2 * (ph - 0.4)), rtol=1e-14)
+    # the ray update: defocus C10 scales the ideal slope by (1 - C10/f)
+    r = O.aberrated_lens(O.make_ray(0.01, -0.02, 0.003, 0.001), 0.5, {"C10": 0.1})
+    assert r["dx"] == pytest.approx((-0.01 / 0.5 + 0.003) * (1 - 0.1 / 0.5), rel=1e-14)
+
+
+def test_krivanek_rotation_covariance():
+    """Rotating the slopes by t and every azimuth by t leaves W unchanged and
+    rotates its gradient."""
+    rng = np.random.default_rng(42)
+    c = _kriv_random(rng)
+    t = 0.7
+    cr = {k: (v + t if k.startswith("phi") else v) for k, v in c.items()}
+    ax, ay = rng.uniform(-0.5, 0.5, 20), rng.uniform(-0.5, 0.5, 20)
+    bx, by = np.cos(t) * ax - np.sin(t) * ay, np.sin(t) * ax + np.cos(t) * ay
+    W0 = O.krivanek_W(np.hypot(ax, ay), np.arctan2(ay, ax), c)
+    W1 = O.krivanek_W(np.hypot(bx, by), np.arctan2(by, bx), cr)
+    np.testing.assert_allclose(W1, W0, rtol=1e-12, atol=1e-14)
+    gx, gy = O.krivanek_grad(ax, ay, c)
+    hx, hy = O.krivanek_grad(bx, by, cr)
+    np.testing.assert_allclose(hx, np.cos(t) * gx - np.sin(t) * gy, rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(hy, np.sin(t) * gx + np.cos(t) * gy, rtol=1e-11, atol=1e-13)
+
+
+def test_krivanek_planted_mistake_is_caught():
+    """The central-difference check fails a gradient whose azimuthal part
+    has the wrong sign (a plausible slip in the dW/dphi mapping)."""
+    rng = np.random.default_rng(43)
+    c = {"C23": 1.0, "phi23": 0.3}
+    ax, ay = rng.uniform(-0.5, 0.5, 10), rng.uniform(-0.5, 0.5, 10)
+    h = 1e-5
+    fd = (O.krivanek_W(np.hypot(ax + h, ay), np.arctan2(ay, ax + h), c)
+          - O.krivanek_W(np.hypot(ax - h, ay), np.arctan2(ay, ax - h), c)) / (2 * h)
+    gx, gy = O.krivanek_grad(ax, ay, c)
+    assert np.allclose(gx, fd, rtol=1e-6, atol=1e-9)
+    a = np.hypot(ax, ay)
+    dW_da, dW_dphi = (gx * ax + gy * ay) / a, gy * ax - gx * ay
+    gx_bad = dW_da * ax / a + dW_dphi * ay / a ** 2
+    assert not np.allclose(gx_bad, fd, rtol=1e-6, atol=1e-9)
