@@ -356,6 +356,37 @@ int tg_gauss_fit_run(const tg_fit_plan* plan, void* workspace, const double* par
  * No reference counterpart (measurement only). */
 int tg_hbm_read_probe(const void* src, int64_t bytes, float* block_sums, int32_t n_blocks, void* stream);
 
+/* ======================================================================== *
+ * Ray sources generated on the device (reference source.py:82-188,
+ * utils.py:46-80, :117-205, :297-325).  Outputs are n-element device arrays
+ * of `dtype` (tg_dtype); tables are device arrays.
+ * kind 0 = PointSource: x = off_x, y = off_y, (dx, dy) = (r cos a, r sin a);
+ * kind 1 = ParallelBeam: (x, y) = (r cos a + off_x, r sin a + off_y), dx = dy = 0.
+ * ======================================================================== */
+
+/* Deterministic concentric rings: replaces concentric_rings with its
+ * multi_cumsum_inplace angles (utils.py:117-175, :46-80) behind
+ * generate_array(random=False).  ring_radius/ring_step [n_rings]: radius and
+ * angle step of each ring; ring_first [n_rings]: first point of each ring
+ * (ring_first[0] = 0); seg_first [n_seg]: first point of each run of the
+ * reference's cumulative angle sum (seg_first[0] = 0).  Point v gets the f64
+ * sum, in order, of the steps of points seg+1..v of its run. */
+int tg_source_rings(const double* ring_radius, const double* ring_step, const int64_t* ring_first, int32_t n_rings,
+                    const int64_t* seg_first, int32_t n_seg, int64_t n, int32_t kind, double off_x, double off_y,
+                    int32_t dtype, void* x, void* y, void* dx, void* dy, void* stream);
+
+/* Fibonacci spiral (utils.py:297-325): r_i = radius sqrt((i + 0.5) / denom)
+ * for i <= band, radius beyond, r_0 = 0; phi_i = i * ga;
+ * x = r cos phi, y = r sin phi. */
+int tg_source_fibonacci(int64_t n, double radius, double band, double denom, double ga, int32_t dtype, void* x,
+                        void* y, void* stream);
+
+/* Uniform points in a disc of radius `scale` (replaces random_coords,
+ * utils.py:178-205, behind generate_array(random=True)): exactly n points from
+ * a counter-based Philox4x32-10 stream keyed by `seed`, r = scale sqrt(u). */
+int tg_source_disc(int64_t n, uint64_t seed, double scale, int32_t kind, double off_x, double off_y, int32_t dtype,
+                   void* x, void* y, void* dx, void* dy, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

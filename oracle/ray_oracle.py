@@ -171,6 +171,66 @@ def aberrated_lens(r, f, c):
                 one=r["one"] * 1.0)
 
 
+def multi_cumsum_inplace(values, partitions, start):
+    """Partitioned cumulative sum, restated loop for loop (reference utils.py:46-80)."""
+    part_idx = 0
+    current = partitions[part_idx]
+    count = 0
+    values[0] = start
+    for v in range(1, values.size):
+        if current == count:
+            count = 0
+            part_idx += 1
+            current = partitions[part_idx]
+            values[v] = start
+        else:
+            values[v] += values[v - 1]
+            count += 1
+
+
+def ring_params(num_points_approx, radius):
+    """(radius, angle) of every point of concentric_rings (reference utils.py:117-175)."""
+    num_rings = max(1, int(np.floor((-1 + np.sqrt(1 + 4 * num_points_approx / np.pi)) / 2)))
+    kth = np.round(2 * np.pi * np.arange(1, num_rings + 1)).astype(int)
+    num_rings = kth.size
+    per_unit = num_points_approx / kth.sum()
+    per_ring = np.round(kth * per_unit).astype(int)
+    radii = np.linspace(0, radius, num_rings + 1, endpoint=True)[1:]
+    with np.errstate(divide="ignore"):
+        div_angle = 2 * np.pi / per_ring
+    params = np.repeat(np.stack((radii, div_angle), axis=0), per_ring.tolist(), axis=-1)
+    multi_cumsum_inplace(params[1, :], per_ring, 0.0)
+    return params[0, :], params[1, :]
+
+
+def concentric_rings(num_points_approx, radius):
+    """(y, x) = (r sin a, r cos a) per point (reference utils.py:117-175)."""
+    r, a = ring_params(num_points_approx, radius)
+    return np.stack((r * np.sin(a), r * np.cos(a)), axis=-1)
+
+
+def point_source_array(num, semi_conv, offset_xy=(0.0, 0.0)):
+    """PointSource.generate_array(random=False) rows [x, y, dx, dy, 1] (reference source.py:104-135)."""
+    dy, dx = concentric_rings(num, semi_conv).T
+    r = np.zeros((dx.size, 5))
+    r[:, 0] += offset_xy[0]
+    r[:, 1] += offset_xy[1]
+    r[:, 2] = dx
+    r[:, 3] = dy
+    r[:, 4] = 1.0
+    return r
+
+
+def parallel_beam_array(num, radius, offset_xy=(0.0, 0.0)):
+    """ParallelBeam.generate_array(random=False) rows (reference source.py:156-188)."""
+    y, x = concentric_rings(num, radius).T
+    r = np.zeros((x.size, 5))
+    r[:, 0] = x + offset_xy[0]
+    r[:, 1] = y + offset_xy[1]
+    r[:, 4] = 1.0
+    return r
+
+
 def fibonacci_spiral(nb_samples, radius, alpha=2):
     """Fibonacci-spiral disc points (reference utils.py:297-325)."""
     ga = np.pi * (3.0 - np.sqrt(5.0))
