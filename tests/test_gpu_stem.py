@@ -189,3 +189,29 @@ def test_fit_launch_variants_are_bitwise_equal(device):
         autotune.clear()
     assert torch.equal(outs[0][0], outs[1][0])
     assert torch.equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("scan_shape", [(12, 10), (65, 64), (1, 1)])
+def test_reduction_sums_partials_and_is_repeatable(device, scan_shape):
+    """The fixed-order FP64 reduction over scan positions equals the sum of
+    the per-scan-position terms (small, ragged and large scans), and
+    repeated calls on one workspace are bitwise equal."""
+    from temgymcore_amd import autotune
+
+    model, _ = stem.reference_workload(n_cam=2, scan_shape=scan_shape, n_side=8)
+    params = stem.params_tensor(model, device)
+    targets = stem.stem_forward(stem.perturbed(model))["coords"]
+    ws = stem.STEMFitWorkspace(model, device)
+    key = (model.n_scan, model.n_rays, model.n_cam)
+    try:
+        for variant in (autotune.STEM_FIT_SCAN, autotune.STEM_FIT_CAMERA):
+            autotune.set_choice(autotune.STEM_FIT, key, variant)
+            outs = [ws.run(params, targets).clone() for _ in range(3)]
+            assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
+            rows = ws.partials.double().cpu().numpy()
+            ref = rows.sum(axis=1)
+            scale = np.abs(rows).sum(axis=1)
+            got = outs[0].cpu().numpy()
+            assert np.all(np.abs(got - ref) <= 1e-13 * scale + 1e-300)
+    finally:
+        autotune.clear()
