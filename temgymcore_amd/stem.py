@@ -318,3 +318,91 @@ def perturbed(model: STEMModel, seed: int = 3, rel: float = 1e-3) -> STEMModel:
     """Every parameter scaled by (1 + rel * N(0, 1)) (target generation)."""
     p = model.params()
     return model.with_params(p * (1.0 + rel * np.random.default_rng(seed).normal(size=p.shape)))
+
+
+# ---------------------------------------------------------------------------
+# calibration loop (the consumer of the fused fit kernel)
+# ---------------------------------------------------------------------------
+FIT_GROUPS = {
+    "descan": slice(0, 16),
+    "scan_step": slice(16, 17),
+    "scan_rotation": slice(17, 18),
+    "camera_lengths": slice(18, None),
+}
+# typical size of a parameter whose start value is 0 (descan entries are
+# dimensionless map coefficients, ~1e-3)
+_ZERO_SCALE = {"descan": 1e-3, "scan_step": 1e-10, "scan_rotation": 1.0, "camera_lengths": 0.1}
+
+
+@dataclasses.dataclass
+class CalibrationResult:
+    model: STEMModel           # the model with the fitted parameters
+    params: np.ndarray         # fitted parameter vector (descan 16, step, rotation, camera lengths)
+    loss: float                # final loss (sum of squared detector residuals, m^2)
+    initial_loss: float
+    evaluations: int           # fused loss + gradient launches
+    history: list              # loss at every evaluation
+
+
+def calibrate(model: STEMModel, targets: torch.Tensor, fit=("descan", "scan_step", "scan_rotation", "camera_lengths"),
+              max_iter: int = 200, history_size: int = 30, params0=None) -> CalibrationResult:
+    """Fit descan map, scan step, scan rotation and camera lengths of `model`
+    to measured detector coordinates (the descan-error calibration over
+    camera lengths that the reference describes, README.md:13-16).
+
+    targets: [n_cam, S, J, 2] f32 device tensor of measured detector (x, y).
+    fit: which parameter groups move (FIT_GROUPS); the rest stay at the
+    model's values (or params0).  Each step evaluates the loss and its full
+    gradient in one pass of the fused fit kernel.  The optimiser is L-BFGS
+    with a strong-Wolfe line search over parameters normalised by their
+    start values (or a typical size where they start at 0), on the loss
+    divided by its start value.
+
+    Detector coordinates are linear in the scan positions, so with the
+    descan map free the scan step and rotation are not determined by them
+    (a change of either is absorbed by the map), and neither are map columns
+    that are proportional for this scan (position and tilt columns when the
+    tilt is pivot x position): L-BFGS then returns one minimiser of that
+    valley.  Fit step and rotation with the map held, or the map with them held."""
+    dev = targets.device
+    p_start = np.asarray(model.params() if params0 is None else params0, dtype=np.float64)
+    idx = []
+    scale = np.zeros_like(p_start)
+    for name in fit:
+        if name not in FIT_GROUPS:
+            raise ValueError(f"unknown parameter group {name!r}; choose from {tuple(FIT_GROUPS)}")
+        sl = FIT_GROUPS[name]
+        ids = list(range(len(p_start)))[sl]
+        idx += ids
+        for i in ids:
+            scale[i] = abs(p_start[i]) if p_start[i] != 0 else _ZERO_SCALE[name]
+    idx = sorted(set(idx))
+    ws = STEMFitWorkspace(model, dev)
+    p0 = torch.from_numpy(p_start).to(dev)
+    sel = torch.tensor(idx, dtype=torch.long, device=dev)
+    sc = torch.from_numpy(scale[idx]).to(dev)
+    delta = torch.zeros(len(idx), dtype=torch.float64, device=dev, requires_grad=True)
+
+    def params_of(d):
+        return p0.index_add(0, sel, sc * d)
+
+    with torch.no_grad():
+        loss0 = float(ws.run(p0, targets)[-1])
+    norm = loss0 if loss0 > 0 else 1.0
+    opt = torch.optim.LBFGS([delta], lr=1.0, max_iter=max_iter, history_size=history_size,
+                            line_search_fn="strong_wolfe", tolerance_grad=1e-15, tolerance_change=1e-15)
+    history = []
+
+    def closure():
+        opt.zero_grad()
+        loss = descan_fit_loss(model, params_of(delta), targets, ws) / norm
+        loss.backward()
+        history.append(float(loss.detach()) * norm)
+        return loss
+
+    opt.step(closure)
+    with torch.no_grad():
+        p = params_of(delta).cpu().numpy()
+        final = float(ws.run(params_of(delta), targets)[-1])
+    return CalibrationResult(model=model.with_params(p), params=p, loss=final, initial_loss=loss0,
+                             evaluations=len(history), history=history)
