@@ -350,6 +350,65 @@ def bench_stem_fwd(steps: int, warmup: int):
     )
 
 
+def bench_backproject(steps: int, warmup: int):
+    import numpy as np
+    import torch
+
+    from oracle import ray_oracle as R
+    from oracle import stem_oracle as SO
+    from temgymcore_amd import stem
+
+    dev = torch.device("cuda", 0)
+    model, sample = stem.reference_workload(n_cam=1)
+    params = stem.params_tensor(model, dev)
+    data = stem.stem_forward(model, params, torch.from_numpy(sample).to(dev), coords=False,
+                             image4d=True)["image4d"]  # 1 GiB f32 shadow images
+    units = model.n_scan * model.n_rays
+    img = torch.zeros(tuple(model.sample_shape), dtype=torch.float64, device=dev)
+    g = model.geometry()
+    lib = stem._lib.load()
+
+    def step():
+        img.zero_()
+        st = lib.tg_stem_backproject(stem.ctypes.byref(g), params.data_ptr(), data.data_ptr(), 0, img.data_ptr(),
+                                     None, stem._lib.stream_ptr(dev))
+        stem._lib.check(st, "tg_stem_backproject")
+
+    ms_per_step, kern_ms, graph_ms = _time_steps(step, steps, warmup)
+    # algorithmic bytes: the one f32 detector value each ray reads
+    bytes_per_launch = 4 * units
+    achieved = bytes_per_launch / (kern_ms * 1e-3)
+    cpu = None
+    if not SKIP_CPU:
+        cores, cpu_model = _cpu_info()
+        small, _ = stem.reference_workload(n_cam=1, scan_shape=(4, 4))
+        geom = dict(scan_shape=small.scan_shape, n_side=small.n_side, semi_conv=small.semi_conv,
+                    defocus=small.defocus, pivot=small.pivot, offsets=small.descan_offsets,
+                    sample_shape=tuple(small.sample_shape))
+        D = data[:small.n_scan].double().cpu().numpy()
+        dT = R.metres_to_pixels_matrix((0.0, 0.0), (small.det_pixel,) * 2, small.det_shape)
+        iT = R.metres_to_pixels_matrix((0.0, 0.0), (small.sample_pixel,) * 2, small.sample_shape)
+        reps, c0 = 0, time.perf_counter()
+        while time.perf_counter() - c0 < 10.0:
+            SO.backproject(geom, small.params(), D, dT, iT)
+            reps += 1
+        cpu_s = (time.perf_counter() - c0) / reps
+        cpu = {"value": small.n_scan * small.n_rays / cpu_s, "unit": "rays/s", "cores": 1, "kind": "port",
+               "sample": f"numpy f64 oracle, 16 scan positions x 4096 rays ({cpu_model})"}
+    return dict(
+        metric="rays/s (4D-STEM shadow-image back-projection: 16.7M rays of a 1 GiB dataset onto the sample plane)",
+        value=units / (ms_per_step * 1e-3), unit="rays/s", ms_per_step=ms_per_step, dtype="f32 data, f64 sums",
+        config={"workload": "backproject: 64x64 scan, 64x64 rays, 256x256 detector, 512x512 sample",
+                "rays": units, "event_ms_per_step": kern_ms, "graph_replay_ms_per_step": graph_ms,
+                "step": "zero the f64 image + one back-projection launch"},
+        roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                  "frac": achieved / HBM_PEAK, "traffic": None,
+                  "note": "gather of one f32 per ray from scattered detector pixels plus LDS/L2 atomics; "
+                          "algorithmic bytes understate the cache-line traffic"},
+        cpu_baseline=cpu,
+    )
+
+
 def aperture_beamlets(n=131072, seed=4):
     """Seeded beamlet centres uniform in a 10 um diameter disc (radius 5 um)."""
     import numpy as np
@@ -535,7 +594,7 @@ def bench_gauss_fit(steps: int, warmup: int):
     )
 
 
-WORKLOADS = {"stem_fit": bench_stem_fit, "stem_fwd": bench_stem_fwd, "plain": bench_plain, "beamlet": bench_beamlet,
+WORKLOADS = {"stem_fit": bench_stem_fit, "stem_fwd": bench_stem_fwd, "backproject": bench_backproject, "plain": bench_plain, "beamlet": bench_beamlet,
              "gauss_fit": bench_gauss_fit}
 DEFAULT = "stem_fit"
 

@@ -211,6 +211,18 @@ int tg_stem_forward(const tg_stem_geometry* g, const double* params, const float
  * workgroup per scan position; the default when S >= 4 x the CU count) and
  * TG_STEM_FIT_CAMERA (one per scan position and camera length; the default
  * for smaller scans); the tuning knob TG_TUNE_STEM_FIT overrides. */
+/* Shadow-image back-projection: the adjoint of tg_stem_forward's 4D deposit
+ * with respect to the sample image (the consumer of 4D-STEM data the
+ * reference describes, README.md:13-16).  For every ray (s, j) whose detector
+ * pixel for camera `cam` (binned as the deposit bins it) lies on the
+ * detector, image[q] += data4d[s, pixel] * w_q and, if weight is not NULL,
+ * weight[q] += w_q, where w_q are the bilinear weights of sample pixel q at
+ * the ray's sample-plane position.  data4d [S, det_h, det_w] f32; image and
+ * weight [img_h, img_w] f64, accumulated into (the caller zeroes them).  The
+ * FP64 sums are order-dependent only at the last ulp (atomics). */
+int tg_stem_backproject(const tg_stem_geometry* g, const double* params, const float* data4d, int32_t cam,
+                        double* image, double* weight, void* stream);
+
 enum tg_stem_fit_variant { TG_STEM_FIT_SCAN = 1, TG_STEM_FIT_CAMERA = 2 };
 int64_t tg_stem_fit_workspace(const tg_stem_geometry* g);
 int tg_stem_fit(const tg_stem_geometry* g, const double* params, const float* targets, double* workspace,

@@ -145,3 +145,35 @@ def fd_grad(geom, p, targets, scan_idx=None, rel=1e-2, steps=None):
         h = steps[i]
         g[i] = (4.0 * central(i, 0.5 * h) - central(i, h)) / 3.0
     return g
+
+
+def backproject(geom, p, data4d, det_Tinv, img_Tinv, cam=0):
+    """Back-projection of data4d [S, H, W] onto the sample image (float64):
+    every ray whose detector pixel (round half even, int32 cast, in range)
+    lies on the detector adds data4d[s, pixel] times its bilinear weights at
+    the sample-plane position (same weights and bounds as ``bilinear``).
+    Returns (image, weight).  det_Tinv / img_Tinv: metres -> pixels matrices.
+    The adjoint of the forward's 4D deposit of ``bilinear`` values."""
+    xd, yd = forward(geom, p)
+    xd, yd = xd[cam], yd[cam]
+    H, W = data4d.shape[1:]
+    fy, fx = R.transform(yd, xd, det_Tinv)
+    py, px = R.round_to_i32(fy), R.round_to_i32(fx)
+    valid = (py >= 0) & (py < H) & (px >= 0) & (px < W)
+    s_idx = np.broadcast_to(np.arange(xd.shape[0])[:, None], xd.shape)
+    d = np.where(valid, data4d[s_idx, np.clip(py, 0, H - 1), np.clip(px, 0, W - 1)], 0.0)
+    xs, ys = sample_plane(geom, p)
+    gy, gx = R.transform(ys, xs, img_Tinv)
+    y0, x0 = np.floor(gy), np.floor(gx)
+    wy, wx = gy - y0, gx - x0
+    h, w = geom["sample_shape"]
+    inside = valid & (y0 >= -1) & (y0 < h) & (x0 >= -1) & (x0 < w)
+    image, weight = np.zeros((h, w)), np.zeros((h, w))
+    for a in (0, 1):
+        for b in (0, 1):
+            yy, xx = (y0 + a).astype(np.int64), (x0 + b).astype(np.int64)
+            ok = inside & (yy >= 0) & (yy < h) & (xx >= 0) & (xx < w)
+            wgt = (wy if a else 1 - wy) * (wx if b else 1 - wx)
+            np.add.at(image, (yy[ok], xx[ok]), (wgt * d)[ok])
+            np.add.at(weight, (yy[ok], xx[ok]), wgt[ok])
+    return image, weight

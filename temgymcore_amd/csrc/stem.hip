@@ -159,6 +159,113 @@ __global__ __launch_bounds__(kThreads) void stem_forward_kernel(const tg_stem_ge
 }
 
 // ---------------------------------------------------------------------------
+// shadow-image back-projection: the adjoint of the 4D deposit above
+// ---------------------------------------------------------------------------
+// image[q] += sum over rays (s, j) whose detector pixel (camera `cam`) lies on
+// the detector of data4d[s, pixel] * w_q(s, j), with w_q the bilinear weight
+// of sample pixel q at the ray's sample-plane position (the same weights,
+// bounds and detector binning as stem_forward_kernel); weight[q] += w_q.
+// One scan position's sample footprint is a few pixels across
+// (defocus x semi-convergence), so each workgroup accumulates into a
+// kBpTile^2 LDS tile centred on it and flushes that once; contributions
+// outside the tile go to global memory directly.  FP64 accumulation.
+constexpr int kBpTile = 32;
+
+__global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_stem_geometry G,
+                                                                    const double* __restrict__ P,
+                                                                    const float* __restrict__ data, int cam,
+                                                                    double* __restrict__ image,
+                                                                    double* __restrict__ weight) {
+  __shared__ float th[kMaxSide];
+  __shared__ double t_img[kBpTile * kBpTile];
+  __shared__ double t_w[kBpTile * kBpTile];
+  __shared__ float sh_a, sh_bx, sh_by;
+  const int64_t s = blockIdx.y;
+  const int64_t J = (int64_t)G.n_side * G.n_side;
+  for (int k = threadIdx.x; k < G.n_side; k += kThreads) th[k] = slope_of(G, k);
+  for (int k = threadIdx.x; k < kBpTile * kBpTile; k += kThreads) {
+    t_img[k] = 0.0;
+    t_w[k] = 0.0;
+  }
+  const ScanPos p = scan_position(G, P, s);
+  if (threadIdx.x == 0) {
+    const double L = P[18 + cam];
+    double bx, by;
+    offsets_at(G, P, p, L, bx, by);
+    sh_a = (float)(G.defocus + L);
+    sh_bx = (float)bx;
+    sh_by = (float)by;
+  }
+  // tile origin: the footprint centre (zero slope) minus half a tile
+  const double xc = p.spx + p.stx * G.defocus, yc = p.spy + p.sty * G.defocus;
+  const double fyc = G.img_m[0] * yc + G.img_m[1] * xc + G.img_m[2];
+  const double fxc = G.img_m[3] * yc + G.img_m[4] * xc + G.img_m[5];
+  const double oyd = floor(fyc) - kBpTile / 2, oxd = floor(fxc) - kBpTile / 2;
+  // origins far outside the image only route everything to global memory
+  const int oy = (int)fmin(fmax(oyd, -4.0 * kBpTile), (double)G.img_h + 4.0 * kBpTile);
+  const int ox = (int)fmin(fmax(oxd, -4.0 * kBpTile), (double)G.img_w + 4.0 * kBpTile);
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kThreads * kFwdRays;
+  const int n = G.n_side, drow = kThreads / n, dcol = kThreads % n;
+  const int j0 = (int)(base + threadIdx.x);
+  int iy = j0 / n, ix = j0 - (j0 / n) * n;
+#pragma unroll
+  for (int u = 0; u < kFwdRays; ++u, ix += dcol, iy += drow) {
+    if (ix >= n) {
+      ix -= n;
+      ++iy;
+    }
+    const int64_t j = base + (int64_t)u * kThreads + threadIdx.x;
+    if (j >= J) continue;
+    const float tx = th[ix], ty = th[iy];
+    // detector pixel, as in stem_forward_kernel
+    const float xd = fmaf(tx, sh_a, sh_bx), yd = fmaf(ty, sh_a, sh_by);
+    const double dfy = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[0], (double)yd), __dmul_rn(G.det_m[1], (double)xd)),
+                                 G.det_m[2]);
+    const double dfx = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[3], (double)yd), __dmul_rn(G.det_m[4], (double)xd)),
+                                 G.det_m[5]);
+    const int32_t py = tg::xla_f64_to_i32(rint(dfy)), px = tg::xla_f64_to_i32(rint(dfx));
+    if (py < 0 || py >= G.det_h || px < 0 || px >= G.det_w) continue;
+    const double d = (double)data[(s * G.det_h + py) * G.det_w + px];
+    // bilinear weights at the sample-plane position, as in bilinear()
+    const double xs = p.spx + ((double)tx + p.stx) * G.defocus;
+    const double ys = p.spy + ((double)ty + p.sty) * G.defocus;
+    const double fy = G.img_m[0] * ys + G.img_m[1] * xs + G.img_m[2];
+    const double fx = G.img_m[3] * ys + G.img_m[4] * xs + G.img_m[5];
+    const double y0d = floor(fy), x0d = floor(fx);
+    if (!(y0d >= -1.0 && y0d < (double)G.img_h && x0d >= -1.0 && x0d < (double)G.img_w)) continue;
+    const int y0 = (int)y0d, x0 = (int)x0d;
+    const float wy = (float)(fy - y0d), wx = (float)(fx - x0d);
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int yy = y0 + a, xx = x0 + b;
+        if (yy < 0 || yy >= G.img_h || xx < 0 || xx >= G.img_w) continue;
+        const double w = (double)((a ? wy : 1.0f - wy) * (b ? wx : 1.0f - wx));
+        const int ty_ = yy - oy, tx_ = xx - ox;
+        if (ty_ >= 0 && ty_ < kBpTile && tx_ >= 0 && tx_ < kBpTile) {
+          atomicAdd(&t_img[ty_ * kBpTile + tx_], w * d);
+          if (weight != nullptr) atomicAdd(&t_w[ty_ * kBpTile + tx_], w);
+        } else {
+          const int64_t q = (int64_t)yy * G.img_w + xx;
+          unsafeAtomicAdd(image + q, w * d);
+          if (weight != nullptr) unsafeAtomicAdd(weight + q, w);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < kBpTile * kBpTile; k += kThreads) {
+    const int yy = oy + k / kBpTile, xx = ox + k % kBpTile;
+    if (yy < 0 || yy >= G.img_h || xx < 0 || xx >= G.img_w) continue;  // never written
+    const int64_t q = (int64_t)yy * G.img_w + xx;
+    if (t_img[k] != 0.0) unsafeAtomicAdd(image + q, t_img[k]);
+    if (weight != nullptr && t_w[k] != 0.0) unsafeAtomicAdd(weight + q, t_w[k]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // fit: forward + adjoint
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float wave_sum(float v) {
@@ -487,6 +594,23 @@ extern "C" int tg_stem_forward(const tg_stem_geometry* g, const double* params, 
   hipLaunchKernelGGL(stem_forward_kernel, grid, dim3(kThreads), 0, (hipStream_t)stream, *g, params, image, coords,
                      values, image4d);
   TG_LAUNCH_CHECK("stem_forward_kernel launch");
+  return TG_OK;
+}
+
+extern "C" int tg_stem_backproject(const tg_stem_geometry* g, const double* params, const float* data4d,
+                                   int32_t cam, double* image, double* weight, void* stream) {
+  tg::clear_error();
+  int st = validate(g, params);
+  if (st != TG_OK) return st;
+  TG_REQUIRE(data4d != nullptr && image != nullptr, "tg_stem_backproject: NULL data or image");
+  TG_REQUIRE(cam >= 0 && cam < g->n_cam, "tg_stem_backproject: camera %d out of range", cam);
+  TG_REQUIRE(g->img_h > 0 && g->img_w > 0 && g->det_h > 0 && g->det_w > 0,
+             "tg_stem_backproject: bad sample or detector shape");
+  const int64_t J = (int64_t)g->n_side * g->n_side;
+  dim3 grid((unsigned)tg::cdiv(J, (int64_t)kThreads * kFwdRays), (unsigned)((int64_t)g->scan_h * g->scan_w));
+  hipLaunchKernelGGL(stem_backproject_kernel, grid, dim3(kThreads), 0, (hipStream_t)stream, *g, params, data4d, cam,
+                     image, weight);
+  TG_LAUNCH_CHECK("stem_backproject_kernel launch");
   return TG_OK;
 }
 

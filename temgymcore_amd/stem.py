@@ -49,6 +49,7 @@ _lib.register_signatures({
     "tg_stem_forward": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, _P, _P, _P, _P]),
     "tg_stem_fit_workspace": (ctypes.c_int64, [ctypes.POINTER(StemGeometry)]),
     "tg_stem_fit": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, _P, _P, _P]),
+    "tg_stem_backproject": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, ctypes.c_int32, _P, _P, _P]),
 })
 
 MAX_CAM = 16
@@ -240,6 +241,33 @@ def stem_forward(model: STEMModel, params: torch.Tensor | None = None, sample: t
                              _ptr(res.get("image4d") if image4d else None), _lib.stream_ptr(device))
     _lib.check(st, "tg_stem_forward")
     return res
+
+
+def backproject(model: STEMModel, data4d: torch.Tensor, params: torch.Tensor | None = None, cam: int = 0,
+                weights: bool = False):
+    """Shadow-image back-projection of a 4D-STEM dataset onto the sample plane.
+
+    data4d: [S, det_h, det_w] f32 device tensor (one detector image per scan
+    position, camera length `cam`).  Returns (image, weight): f64
+    [img_h, img_w] sums over every ray of the detector value it lands on
+    times the bilinear weights at its sample-plane position (the adjoint of
+    the forward's 4D deposit), and the sums of those weights (or None).
+    image / weight is the mean back-projected value per sample pixel; it
+    lines up across scan positions only when the model's geometry is right."""
+    device = _lib.require_device()
+    params = params_tensor(model, device) if params is None else params
+    _check_params(model, params)
+    shape = (model.n_scan,) + tuple(model.det_shape)
+    if tuple(data4d.shape) != shape or data4d.dtype != torch.float32 or not data4d.is_cuda:
+        raise ValueError(f"data4d must be a device float32 tensor of shape {shape}")
+    data4d = data4d.contiguous()
+    image = torch.zeros(tuple(model.sample_shape), dtype=torch.float64, device=device)
+    weight = torch.zeros_like(image) if weights else None
+    g = model.geometry()
+    st = _lib.load().tg_stem_backproject(ctypes.byref(g), params.data_ptr(), data4d.data_ptr(), int(cam),
+                                         image.data_ptr(), _ptr(weight), _lib.stream_ptr(device))
+    _lib.check(st, "tg_stem_backproject")
+    return image, weight
 
 
 def _check_params(model, params):

@@ -256,3 +256,47 @@ def test_calibration_recovers_perturbed_geometry(device):
     np.testing.assert_array_equal(res.params[:16], t[:16])
     with pytest.raises(ValueError):
         stem.calibrate(model, targets, fit=("pivot",))
+
+
+def _bp_oracle(model, D_host, cam=0):
+    geom = dict(_geom(model), sample_shape=tuple(model.sample_shape))
+    det_Tinv = R.metres_to_pixels_matrix((0.0, 0.0), (model.det_pixel,) * 2, model.det_shape)
+    img_Tinv = R.metres_to_pixels_matrix((0.0, 0.0), (model.sample_pixel,) * 2, model.sample_shape)
+    return SO.backproject(geom, model.params(), D_host, det_Tinv, img_Tinv, cam)
+
+
+@pytest.mark.parametrize("defocus", [200e-9, 2e-6])
+def test_backproject_vs_oracle(device, defocus):
+    """Constant detector images per scan position (so the detector binning of
+    a ray on a pixel edge cannot matter) back-project to the oracle's sums.
+    At 2 um defocus the footprints exceed the workgroup's LDS tile and the
+    global-memory path carries the outer rays.  Tolerance: the kernel's
+    bilinear weights are f32 products (as in the forward), 1e-6 relative."""
+    model, _ = stem.reference_workload(n_cam=2, scan_shape=(8, 8), n_side=16, defocus=defocus, det_pixel=1e-5)
+    rng = np.random.default_rng(12)
+    c = rng.uniform(0.5, 2.0, model.n_scan)
+    D = np.broadcast_to(c[:, None, None], (model.n_scan,) + tuple(model.det_shape)).astype(np.float32)
+    for cam in (0, 1):
+        img, w = stem.backproject(model, torch.from_numpy(np.ascontiguousarray(D)).to(device), cam=cam, weights=True)
+        ref_img, ref_w = _bp_oracle(model, D.astype(np.float64), cam)
+        assert ref_w.sum() > 0.99 * model.n_scan * model.n_rays  # every ray lands
+        np.testing.assert_allclose(w.cpu().numpy(), ref_w, rtol=0, atol=1e-6 * ref_w.max())
+        np.testing.assert_allclose(img.cpu().numpy(), ref_img, rtol=0, atol=1e-6 * 2.0 * ref_w.max())
+    img2, none = stem.backproject(model, torch.from_numpy(np.ascontiguousarray(D)).to(device))
+    assert none is None and img2.shape == tuple(model.sample_shape)
+
+
+def test_backproject_is_the_adjoint_of_the_forward_deposit(device):
+    """<F I, D> == <I, F^T D> with F the forward kernel's 4D deposit (camera
+    0) and F^T the back-projection kernel, for random I and D (the forward
+    sums in f32: 1e-5 relative)."""
+    model, _ = stem.reference_workload(n_cam=1, scan_shape=(16, 16), n_side=32)
+    rng = np.random.default_rng(13)
+    I = torch.from_numpy(rng.normal(size=model.sample_shape).astype(np.float32)).to(device)
+    D = torch.from_numpy(rng.normal(size=(model.n_scan,) + tuple(model.det_shape)).astype(np.float32)).to(device)
+    FI = stem.stem_forward(model, sample=I, coords=False, image4d=True)["image4d"]
+    lhs = float((FI.double() * D.double()).sum())
+    B, _ = stem.backproject(model, D)
+    rhs = float((I.double() * B).sum())
+    scale = float((FI.double().abs() * D.double().abs()).sum())
+    assert abs(lhs - rhs) <= 1e-5 * scale, (lhs, rhs, scale)

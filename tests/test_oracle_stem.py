@@ -108,3 +108,55 @@ def test_bilinear_identities():
     assert S.bilinear(img, Tinv, mx, my)[0] == pytest.approx(img[3, 11], abs=1e-12)
     # outside the image: zero
     assert S.bilinear(img, Tinv, np.array([1e-6]), np.array([0.0]))[0] == 0.0
+
+
+def _bp_setup(seed=9):
+    geom = dict(scan_shape=(6, 5), n_side=8, semi_conv=0.5e-3, defocus=200e-9, pivot=1e4,
+                offsets=(0.0, 0.0, 0.0, 0.0), sample_shape=(40, 44))
+    rng = np.random.default_rng(seed)
+    p = np.concatenate([rng.normal(0, 1e-3, 16), [1e-10, 12.0], [0.5, 0.8]])
+    det_Tinv = R.metres_to_pixels_matrix((0.0, 0.0), (2e-5, 2e-5), (64, 60))
+    img_Tinv = R.metres_to_pixels_matrix((0.0, 0.0), (2e-11, 2e-11), geom["sample_shape"])
+    return geom, p, det_Tinv, img_Tinv, rng
+
+
+def _deposit(geom, p, img, det_Tinv, img_Tinv, shape, cam=0):
+    """4D deposit of bilinear sample values into [S, H, W], from the pinned
+    forward, pixel transform and bilinear lookup."""
+    xd, yd = S.forward(geom, p)
+    xs, ys = S.sample_plane(geom, p)
+    v = S.bilinear(img, img_Tinv, xs, ys)
+    out = np.zeros((xd.shape[1],) + shape)
+    for s in range(xd.shape[1]):
+        py, px = R.metres_to_pixels(xd[cam, s], yd[cam, s], det_Tinv)
+        ok = (py >= 0) & (py < shape[0]) & (px >= 0) & (px < shape[1])
+        np.add.at(out[s], (py[ok], px[ok]), v[s][ok])
+    return out
+
+
+def test_backproject_is_the_transpose_of_the_deposit():
+    """<deposit(I), D> == <I, backproject(D)> for random I, D and both
+    cameras; a transposed sample-pixel transform breaks it."""
+    geom, p, det_Tinv, img_Tinv, rng = _bp_setup()
+    for cam in (0, 1):
+        img = rng.normal(size=geom["sample_shape"])
+        D = rng.normal(size=(30, 64, 60))
+        lhs = np.sum(_deposit(geom, p, img, det_Tinv, img_Tinv, (64, 60), cam) * D)
+        B, Wt = S.backproject(geom, p, D, det_Tinv, img_Tinv, cam)
+        assert abs(np.sum(img * B) - lhs) <= 1e-12 * np.sum(np.abs(img) * np.abs(B)) + 1e-300
+        assert np.all(Wt >= 0) and Wt.sum() > 0
+    swapped = img_Tinv[[1, 0, 2]]
+    B_bad, _ = S.backproject(geom, p, D, det_Tinv, swapped, 1)
+    assert abs(np.sum(img * B_bad) - lhs) > 1e-3 * abs(lhs)
+
+
+def test_backproject_weights_sum_to_ray_count():
+    """Bilinear weights of a ray sum to 1 when all four neighbours are on the
+    image, so the weight image sums to the number of rays that land on both
+    the detector and the image interior."""
+    geom, p, det_Tinv, img_Tinv, rng = _bp_setup(10)
+    D = np.ones((30, 64, 60))
+    B, Wt = S.backproject(geom, p, D, det_Tinv, img_Tinv, 0)
+    np.testing.assert_allclose(B, Wt, rtol=1e-15, atol=0)
+    n_rays = 30 * 64
+    assert abs(Wt.sum() - n_rays) < 1e-9 * n_rays
