@@ -223,6 +223,14 @@ int tg_stem_forward(const tg_stem_geometry* g, const double* params, const float
 int tg_stem_backproject(const tg_stem_geometry* g, const double* params, const float* data4d, int32_t cam,
                         double* image, double* weight, void* stream);
 
+/* The launches of tg_stem_fit before its final reduction: fills the head of
+ * the workspace with the [18 + n_cam + 1, S] per-scan-position terms (row t,
+ * column s: scan position s's contribution to out[t]; the last row is its
+ * share of the loss, a residual map over the scan).  tg_stem_fit = this +
+ * the fixed-order FP64 sums over s. */
+int tg_stem_fit_terms(const tg_stem_geometry* g, const double* params, const float* targets, double* workspace,
+                      void* stream);
+
 enum tg_stem_fit_variant { TG_STEM_FIT_SCAN = 1, TG_STEM_FIT_CAMERA = 2 };
 int64_t tg_stem_fit_workspace(const tg_stem_geometry* g);
 int tg_stem_fit(const tg_stem_geometry* g, const double* params, const float* targets, double* workspace,

@@ -68,13 +68,47 @@ __device__ __forceinline__ float slope_of(const tg_stem_geometry& G, int k) {
 
 // Bilinear lookup of the sample image at (x, y) metres; neighbours outside
 // the image contribute zero.
-__device__ __forceinline__ float bilinear(const tg_stem_geometry& G, const float* __restrict__ img, double x,
-                                          double y) {
-  const double fy = G.img_m[0] * y + G.img_m[1] * x + G.img_m[2];
-  const double fx = G.img_m[3] * y + G.img_m[4] * x + G.img_m[5];
+// Sample-image pixel coordinates of the sample-plane position of slope
+// (tx, ty) at one scan position.  The position is affine in the slope,
+//   x = spx + (tx + stx) d,  y = spy + (ty + sty) d  (d = defocus),
+// so (fy, fx) = origin + slope terms, with the origin and the four slope
+// coefficients folded per scan position (FP64).
+struct SampleMap {
+  double fy0, fx0, ky_x, ky_y, kx_x, kx_y;
+};
+
+__device__ __forceinline__ SampleMap sample_map(const tg_stem_geometry& G, const ScanPos& p) {
+  const double xc = p.spx + p.stx * G.defocus, yc = p.spy + p.sty * G.defocus;
+  SampleMap m;
+  m.fy0 = G.img_m[0] * yc + G.img_m[1] * xc + G.img_m[2];
+  m.fx0 = G.img_m[3] * yc + G.img_m[4] * xc + G.img_m[5];
+  m.ky_y = G.img_m[0] * G.defocus;
+  m.ky_x = G.img_m[1] * G.defocus;
+  m.kx_y = G.img_m[3] * G.defocus;
+  m.kx_x = G.img_m[4] * G.defocus;
+  return m;
+}
+
+__device__ __forceinline__ void sample_pixel(const SampleMap& m, float tx, float ty, double& fy, double& fx) {
+  fy = fma(m.ky_y, (double)ty, fma(m.ky_x, (double)tx, m.fy0));
+  fx = fma(m.kx_y, (double)ty, fma(m.kx_x, (double)tx, m.fx0));
+}
+
+// The sample-image region one scan position's rays read: kImgTile^2 pixels
+// around the footprint centre, staged in LDS (pixels outside the image are
+// never read from it: the lookup keeps the image-bounds test).
+constexpr int kImgTile = 32;
+
+// Bilinear lookup at pixel coordinates (fy, fx); neighbours outside the
+// image contribute zero (the lookup is 0 when (floor fy, floor fx) is not in
+// [-1, h) x [-1, w)).  The four neighbours come from the LDS tile when they
+// all lie in it, else from global memory (same weights, same order).
+__device__ __forceinline__ float bilinear_tiled(const tg_stem_geometry& G, const float* __restrict__ img,
+                                                const float* tile, int oy, int ox, double fy, double fx) {
   const double y0d = floor(fy), x0d = floor(fx);
   if (!(y0d >= -1.0 && y0d < (double)G.img_h && x0d >= -1.0 && x0d < (double)G.img_w)) return 0.0f;
   const int y0 = (int)y0d, x0 = (int)x0d;
+  const bool in_tile = y0 >= oy && y0 + 1 < oy + kImgTile && x0 >= ox && x0 + 1 < ox + kImgTile;
   const float wy = (float)(fy - y0d), wx = (float)(fx - x0d);
   float acc = 0.0f;
 #pragma unroll
@@ -84,7 +118,8 @@ __device__ __forceinline__ float bilinear(const tg_stem_geometry& G, const float
       const int yy = y0 + a, xx = x0 + b;
       if (yy < 0 || yy >= G.img_h || xx < 0 || xx >= G.img_w) continue;
       const float w = (a ? wy : 1.0f - wy) * (b ? wx : 1.0f - wx);
-      acc = fmaf(w, img[(int64_t)yy * G.img_w + xx], acc);
+      const float v = in_tile ? tile[(yy - oy) * kImgTile + (xx - ox)] : img[(int64_t)yy * G.img_w + xx];
+      acc = fmaf(w, v, acc);
     }
   }
   return acc;
@@ -93,7 +128,10 @@ __device__ __forceinline__ float bilinear(const tg_stem_geometry& G, const float
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
-constexpr int kFwdRays = 4;  // rays per lane
+// rays per lane: a workgroup covers 4096 rays (one scan position of the
+// 64x64 workload), so its set-up (slopes, scan position, camera offsets)
+// is paid once per 4096 rays rather than per 1024
+constexpr int kFwdRays = 16;
 
 __global__ __launch_bounds__(kThreads) void stem_forward_kernel(const tg_stem_geometry G, const double* __restrict__ P,
                                                                 const float* __restrict__ img,
@@ -102,11 +140,23 @@ __global__ __launch_bounds__(kThreads) void stem_forward_kernel(const tg_stem_ge
                                                                 float* __restrict__ image4d) {
   __shared__ float th[kMaxSide];
   __shared__ float sh_a[TG_STEM_MAX_CAM], sh_bx[TG_STEM_MAX_CAM], sh_by[TG_STEM_MAX_CAM];
+  __shared__ float tile[kImgTile * kImgTile];
   const int64_t s = blockIdx.y;
   const int64_t S = (int64_t)G.scan_h * G.scan_w;
   const int64_t J = (int64_t)G.n_side * G.n_side;
   for (int k = threadIdx.x; k < G.n_side; k += kThreads) th[k] = slope_of(G, k);
   const ScanPos p = scan_position(G, P, s);
+  const SampleMap smap = sample_map(G, p);
+  // the image tile around the footprint centre (origins far outside the
+  // image only send every lookup to global memory)
+  const int oy = (int)fmin(fmax(floor(smap.fy0) - kImgTile / 2, -4.0 * kImgTile), (double)G.img_h + 4.0 * kImgTile);
+  const int ox = (int)fmin(fmax(floor(smap.fx0) - kImgTile / 2, -4.0 * kImgTile), (double)G.img_w + 4.0 * kImgTile);
+  if (img != nullptr) {
+    for (int k = threadIdx.x; k < kImgTile * kImgTile; k += kThreads) {
+      const int yy = oy + k / kImgTile, xx = ox + k % kImgTile;
+      tile[k] = (yy >= 0 && yy < G.img_h && xx >= 0 && xx < G.img_w) ? img[(int64_t)yy * G.img_w + xx] : 0.0f;
+    }
+  }
   if (threadIdx.x < G.n_cam) {
     const double L = P[18 + threadIdx.x];
     double bx, by;
@@ -116,43 +166,63 @@ __global__ __launch_bounds__(kThreads) void stem_forward_kernel(const tg_stem_ge
     sh_by[threadIdx.x] = (float)by;
   }
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kThreads * kFwdRays;
+  // the tile lies inside the image: lookups whose neighbours are all in the
+  // tile need no image-bounds tests
+  const bool tile_inside = oy >= 0 && ox >= 0 && oy + kImgTile <= G.img_h && ox + kImgTile <= G.img_w;
+  const int jb = blockIdx.x * kThreads * kFwdRays;  // J <= 2^20
+  // per-workgroup output bases; rays index them with 32-bit offsets
+  float* __restrict__ vrow = values != nullptr ? values + s * J : nullptr;
+  float2* __restrict__ crow = coords != nullptr ? (float2*)coords + s * J : nullptr;
+  const int64_t cstride = S * J;  // between camera lengths, in float2
   // ray j = (row iy, column ix), stepped by kThreads without dividing
   const int n = G.n_side, drow = kThreads / n, dcol = kThreads % n;
-  const int j0 = (int)(base + threadIdx.x);  // J <= 2^20
+  const int j0 = jb + (int)threadIdx.x;
   int iy = j0 / n, ix = j0 - (j0 / n) * n;
-#pragma unroll
+  const bool need_v = values != nullptr || image4d != nullptr;
+#pragma unroll 4
   for (int u = 0; u < kFwdRays; ++u, ix += dcol, iy += drow) {
     if (ix >= n) {
       ix -= n;
       ++iy;
     }
-    const int64_t j = base + (int64_t)u * kThreads + threadIdx.x;
-    if (j >= J) continue;
+    const int j = j0 + u * kThreads;
+    if (j >= (int)J) continue;
     const float tx = th[ix], ty = th[iy];
     float v = 0.0f;
-    if (values != nullptr || image4d != nullptr) {
-      // sample plane position in f64 (scan offsets are ~1e-9 m)
-      const double xs = p.spx + ((double)tx + p.stx) * G.defocus;
-      const double ys = p.spy + ((double)ty + p.sty) * G.defocus;
-      v = bilinear(G, img, xs, ys);
-      if (values != nullptr) values[s * J + j] = v;
+    if (need_v) {
+      double fy, fx;
+      sample_pixel(smap, tx, ty, fy, fx);
+      const double y0d = floor(fy), x0d = floor(fx);
+      // tile coordinates (clamped first: positions far off the image or NaN
+      // only ever take the general path)
+      const int y0 = (int)fmin(fmax(y0d, -1e8), 1e8) - oy, x0 = (int)fmin(fmax(x0d, -1e8), 1e8) - ox;
+      if (tile_inside && y0 >= 0 && y0 < kImgTile - 1 && x0 >= 0 && x0 < kImgTile - 1) {
+        // all four neighbours in the tile (hence in the image): bilinear_tiled's
+        // accumulation without its bounds tests
+        const float wy = (float)(fy - y0d), wx = (float)(fx - x0d);
+        const int t = y0 * kImgTile + x0;
+        v = fmaf((1.0f - wy) * (1.0f - wx), tile[t], 0.0f);
+        v = fmaf((1.0f - wy) * wx, tile[t + 1], v);
+        v = fmaf(wy * (1.0f - wx), tile[t + kImgTile], v);
+        v = fmaf(wy * wx, tile[t + kImgTile + 1], v);
+      } else {
+        v = bilinear_tiled(G, img, tile, oy, ox, fy, fx);
+      }
+      if (vrow != nullptr) vrow[j] = v;
     }
-    for (int c = 0; c < G.n_cam; ++c) {
+    float2* __restrict__ co = crow;
+    for (int c = 0; c < G.n_cam; ++c, co += cstride) {
       const float xd = fmaf(tx, sh_a[c], sh_bx[c]);
       const float yd = fmaf(ty, sh_a[c], sh_by[c]);
-      if (coords != nullptr) {
-        float2* o = (float2*)coords + ((int64_t)c * S + s) * J + j;
-        *o = make_float2(xd, yd);
-      }
+      if (co != nullptr) co[j] = make_float2(xd, yd);
       if (c == 0 && image4d != nullptr) {
         const double fy = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[0], (double)yd), __dmul_rn(G.det_m[1], (double)xd)),
                                     G.det_m[2]);
         const double fx = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[3], (double)yd), __dmul_rn(G.det_m[4], (double)xd)),
                                     G.det_m[5]);
-        const int32_t iy = tg::xla_f64_to_i32(rint(fy)), ix = tg::xla_f64_to_i32(rint(fx));
-        if (iy >= 0 && iy < G.det_h && ix >= 0 && ix < G.det_w)
-          unsafeAtomicAdd(image4d + (s * G.det_h + iy) * G.det_w + ix, v);
+        const int32_t py = tg::xla_f64_to_i32(rint(fy)), px = tg::xla_f64_to_i32(rint(fx));
+        if (py >= 0 && py < G.det_h && px >= 0 && px < G.det_w)
+          unsafeAtomicAdd(image4d + (s * G.det_h + py) * G.det_w + px, v);
       }
     }
   }
@@ -197,10 +267,8 @@ __global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_ste
     sh_by = (float)by;
   }
   // tile origin: the footprint centre (zero slope) minus half a tile
-  const double xc = p.spx + p.stx * G.defocus, yc = p.spy + p.sty * G.defocus;
-  const double fyc = G.img_m[0] * yc + G.img_m[1] * xc + G.img_m[2];
-  const double fxc = G.img_m[3] * yc + G.img_m[4] * xc + G.img_m[5];
-  const double oyd = floor(fyc) - kBpTile / 2, oxd = floor(fxc) - kBpTile / 2;
+  const SampleMap smap = sample_map(G, p);
+  const double oyd = floor(smap.fy0) - kBpTile / 2, oxd = floor(smap.fx0) - kBpTile / 2;
   // origins far outside the image only route everything to global memory
   const int oy = (int)fmin(fmax(oyd, -4.0 * kBpTile), (double)G.img_h + 4.0 * kBpTile);
   const int ox = (int)fmin(fmax(oxd, -4.0 * kBpTile), (double)G.img_w + 4.0 * kBpTile);
@@ -209,7 +277,7 @@ __global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_ste
   const int n = G.n_side, drow = kThreads / n, dcol = kThreads % n;
   const int j0 = (int)(base + threadIdx.x);
   int iy = j0 / n, ix = j0 - (j0 / n) * n;
-#pragma unroll
+#pragma unroll 4
   for (int u = 0; u < kFwdRays; ++u, ix += dcol, iy += drow) {
     if (ix >= n) {
       ix -= n;
@@ -227,11 +295,9 @@ __global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_ste
     const int32_t py = tg::xla_f64_to_i32(rint(dfy)), px = tg::xla_f64_to_i32(rint(dfx));
     if (py < 0 || py >= G.det_h || px < 0 || px >= G.det_w) continue;
     const double d = (double)data[(s * G.det_h + py) * G.det_w + px];
-    // bilinear weights at the sample-plane position, as in bilinear()
-    const double xs = p.spx + ((double)tx + p.stx) * G.defocus;
-    const double ys = p.spy + ((double)ty + p.sty) * G.defocus;
-    const double fy = G.img_m[0] * ys + G.img_m[1] * xs + G.img_m[2];
-    const double fx = G.img_m[3] * ys + G.img_m[4] * xs + G.img_m[5];
+    // bilinear weights at the sample-plane position, as in bilinear_tiled()
+    double fy, fx;
+    sample_pixel(smap, tx, ty, fy, fx);
     const double y0d = floor(fy), x0d = floor(fx);
     if (!(y0d >= -1.0 && y0d < (double)G.img_h && x0d >= -1.0 && x0d < (double)G.img_w)) continue;
     const int y0 = (int)y0d, x0 = (int)x0d;
@@ -620,12 +686,13 @@ extern "C" int64_t tg_stem_fit_workspace(const tg_stem_geometry* g) {
   return (int64_t)sizeof(double) * S * ((18 + g->n_cam + 1) + (int64_t)g->n_cam * kMom);
 }
 
-extern "C" int tg_stem_fit(const tg_stem_geometry* g, const double* params, const float* targets, double* workspace,
-                           double* out, void* stream) {
-  tg::clear_error();
-  int st = validate(g, params);
-  if (st != TG_OK) return st;
-  TG_REQUIRE(targets != nullptr && workspace != nullptr && out != nullptr, "tg_stem_fit: NULL buffer");
+namespace {
+
+// The per-scan-position terms [np, S] at the head of the workspace (the
+// launches of tg_stem_fit before its final reduction).
+int fit_terms(const tg_stem_geometry* g, const double* params, const float* targets, double* workspace,
+              hipStream_t stream) {
+  TG_REQUIRE(targets != nullptr && workspace != nullptr, "tg_stem_fit: NULL buffer");
   TG_REQUIRE(((uintptr_t)targets & 15) == 0, "tg_stem_fit: targets must be 16-byte aligned");
   const int64_t S = (int64_t)g->scan_h * g->scan_w;
   const int np = 18 + g->n_cam + 1;
@@ -650,7 +717,31 @@ extern "C" int tg_stem_fit(const tg_stem_geometry* g, const double* params, cons
                        partials);
     TG_LAUNCH_CHECK("stem_fit_epilogue launch");
   }
-  hipLaunchKernelGGL(row_sum_kernel, dim3((unsigned)np), dim3(kSumThreads), 0, (hipStream_t)stream, partials, S, out);
+  return TG_OK;
+}
+
+}  // namespace
+
+extern "C" int tg_stem_fit_terms(const tg_stem_geometry* g, const double* params, const float* targets,
+                                 double* workspace, void* stream) {
+  tg::clear_error();
+  int st = validate(g, params);
+  if (st != TG_OK) return st;
+  return fit_terms(g, params, targets, workspace, (hipStream_t)stream);
+}
+
+extern "C" int tg_stem_fit(const tg_stem_geometry* g, const double* params, const float* targets, double* workspace,
+                           double* out, void* stream) {
+  tg::clear_error();
+  int st = validate(g, params);
+  if (st != TG_OK) return st;
+  TG_REQUIRE(out != nullptr, "tg_stem_fit: NULL buffer");
+  st = fit_terms(g, params, targets, workspace, (hipStream_t)stream);
+  if (st != TG_OK) return st;
+  const int64_t S = (int64_t)g->scan_h * g->scan_w;
+  const int np = 18 + g->n_cam + 1;
+  hipLaunchKernelGGL(row_sum_kernel, dim3((unsigned)np), dim3(kSumThreads), 0, (hipStream_t)stream, workspace, S,
+                     out);
   TG_LAUNCH_CHECK("row_sum_kernel launch");
   return TG_OK;
 }

@@ -50,6 +50,7 @@ _lib.register_signatures({
     "tg_stem_fit_workspace": (ctypes.c_int64, [ctypes.POINTER(StemGeometry)]),
     "tg_stem_fit": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, _P, _P, _P]),
     "tg_stem_backproject": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, ctypes.c_int32, _P, _P, _P]),
+    "tg_stem_fit_terms": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, _P, _P]),
 })
 
 MAX_CAM = 16
@@ -303,6 +304,21 @@ class STEMFitWorkspace:
                                      _lib.stream_ptr(targets.device))
         _lib.check(st, "tg_stem_fit")
         return self.out
+
+    def terms(self, params: torch.Tensor, targets: torch.Tensor) -> torch.Tensor:
+        """Enqueue the per-scan-position terms only; returns the [np, S] view
+        (row t, column s: scan position s's share of run()'s entry t; the
+        last row is the loss per scan position, a residual map)."""
+        m = self.model
+        _check_params(m, params)
+        shape = (m.n_cam, m.n_scan, m.n_rays, 2)
+        if tuple(targets.shape) != shape or targets.dtype != torch.float32 or not targets.is_cuda \
+                or not targets.is_contiguous():
+            raise ValueError(f"targets must be a contiguous device float32 tensor of shape {shape}")
+        st = _lib.load().tg_stem_fit_terms(ctypes.byref(self.geometry), params.data_ptr(), targets.data_ptr(),
+                                           self.workspace.data_ptr(), _lib.stream_ptr(targets.device))
+        _lib.check(st, "tg_stem_fit_terms")
+        return self.partials
 
 
 class _DescanFitLoss(torch.autograd.Function):

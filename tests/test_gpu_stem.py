@@ -300,3 +300,20 @@ def test_backproject_is_the_adjoint_of_the_forward_deposit(device):
     rhs = float((I.double() * B).sum())
     scale = float((FI.double().abs() * D.double().abs()).sum())
     assert abs(lhs - rhs) <= 1e-5 * scale, (lhs, rhs, scale)
+
+
+def test_fit_terms_are_the_per_scan_position_shares(device):
+    """tg_stem_fit_terms writes the same per-scan-position terms as the full
+    fit; their row sums are the fit's outputs, the last row a loss map."""
+    model, _ = stem.reference_workload(n_cam=2, scan_shape=(10, 12), n_side=16)
+    params = stem.params_tensor(model, device)
+    targets = stem.stem_forward(stem.perturbed(model))["coords"]
+    ws = stem.STEMFitWorkspace(model, device)
+    out = ws.run(params, targets).clone()
+    full = ws.partials.clone()
+    ws.workspace.zero_()
+    terms = ws.terms(params, targets).clone()
+    assert torch.equal(terms, full)
+    rows = terms.cpu().numpy()
+    assert np.all(np.abs(rows.sum(axis=1) - out.cpu().numpy()) <= 1e-12 * np.abs(rows).sum(axis=1) + 1e-300)
+    assert np.all(rows[-1] >= 0)
