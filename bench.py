@@ -358,7 +358,10 @@ def bench_stem_fwd(steps: int, warmup: int):
         config={"workload": "stem_fwd: 64x64 scan, 64x64 rays, camera length 0.5 m", "rays": units,
                 "event_ms_per_step": kern_ms, "graph_replay_ms_per_step": graph_ms},
         roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK, "traffic": None},
+                  "frac": achieved / HBM_PEAK,
+                  "traffic": _profiled_traffic("stem_forward_kernel-stem_fwd_64x64x4096.md"),
+                  "traffic_source": "profiles/stem_forward_kernel-stem_fwd_64x64x4096.md (calibrated FETCH_SIZE + "
+                                    "WRITE_SIZE)"},
         cpu_baseline=None if SKIP_CPU else {
             "value": 16 * model.n_rays / cpu_s, "unit": "rays/s", "cores": 1, "kind": "port",
             "sample": f"numpy f64 oracle, 16 scan positions x 4096 rays ({cpu_model})"},
@@ -417,9 +420,12 @@ def bench_backproject(steps: int, warmup: int):
                 "rays": units, "event_ms_per_step": kern_ms, "graph_replay_ms_per_step": graph_ms,
                 "step": "zero the f64 image + one back-projection launch"},
         roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                  "frac": achieved / HBM_PEAK, "traffic": None,
+                  "frac": achieved / HBM_PEAK,
+                  "traffic": _profiled_traffic("stem_backproject_kernel-backproject_64x64x4096.md"),
+                  "traffic_source": "profiles/stem_backproject_kernel-backproject_64x64x4096.md (calibrated "
+                                    "FETCH_SIZE; 4x the algorithmic bytes: one cache line per gathered value)",
                   "note": "gather of one f32 per ray from scattered detector pixels plus LDS/L2 atomics; "
-                          "algorithmic bytes understate the cache-line traffic"},
+                          "bound by the gather latency and atomics, not HBM bandwidth"},
         cpu_baseline=cpu,
     )
 
