@@ -189,3 +189,29 @@ def test_run_with_grads_aberration_coefficient(device):
         ref = (4 * d2[k] - d1[k]) / 3
         got = getattr(g, k).cpu().numpy()
         assert np.abs(got - ref).max() <= 1e-7 * np.abs(ref).max(), k
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(128, 128), (256, 200)])
+def test_aberrated_lens_with_binning(device, dtype, shape):
+    """Every Krivanek term at once splits the model into two programs; the
+    fused binning of the last, into a small (LDS-histogram size) or large
+    detector, counts exactly the pixels of the positions the launch returns."""
+    from temgymcore_amd.run import run_to_image
+
+    rng = np.random.default_rng(65)
+    n, f = 300_000, 2.5e-4
+    # every coefficient non-zero, at Krivanek-like sizes (order n ~ 10^(2n-9) m)
+    c = {k: (v if k.startswith("phi") else v * 10.0 ** (2 * int(k[1]) - 9)) for k, v in _coeffs(rng, 1.0).items()}
+    host = [rng.uniform(-2e-8, 2e-8, n).astype(dtype) for _ in range(2)]
+    host += [rng.uniform(-1e-2, 1e-2, n).astype(dtype) for _ in range(2)]
+    t = [torch.from_numpy(a).to(device) for a in host]
+    det = Detector(z=2 * f, pixel_size=(5e-8, 5e-8), shape=shape)
+    model = [AberratedLensKrivanek(z=f, focal_length=f, coeffs=c), det]
+    out, img = run_to_image(Ray(x=t[0], y=t[1], dx=t[2], dy=t[3], z=0.0, pathlength=0.0), model)
+    x, y = out.x.cpu().numpy(), out.y.cpu().numpy()
+    Tinv = O.metres_to_pixels_matrix((0.0, 0.0), (5e-8, 5e-8), shape)
+    py, px = O.metres_to_pixels(x, y, Tinv)
+    ref = O.into_image(py, px, shape)
+    assert ref.sum() > 0.5 * n  # most rays land
+    np.testing.assert_array_equal(img.cpu().numpy().astype(np.int64), ref.astype(np.int64))

@@ -209,3 +209,67 @@ def test_direct_component_call_on_device_ray(device):
     for k in ("dx", "dy", "pathlength"):
         np.testing.assert_array_equal(_host(getattr(out, k)), ref[k])
     assert out.x is ray.x  # unchanged fields are passed through
+
+
+def _wide_component():
+    import dataclasses
+
+    from temgymcore_amd.components import Component
+
+    @dataclasses.dataclass(frozen=True)
+    class Mixer(Component):
+        """20 intermediate values live at once: the program needs more than
+        16 registers (the scratch register file)."""
+        z: float
+
+        def __call__(self, ray):
+            terms = [ray.x * (1.0 + 0.125 * k) + ray.dx * (0.5 - 0.0625 * k) for k in range(20)]
+            acc = terms[0]
+            for k in range(1, 20):
+                acc = acc + terms[k] * terms[20 - k]
+            return ray.derive(dx=ray.dx + acc * 0.25, dy=ray.dy - acc * 0.125)
+
+    return Mixer
+
+
+def _mixer_ref(x, dx, dt):
+    terms = [x * dt(1.0 + 0.125 * k) + dx * dt(0.5 - 0.0625 * k) for k in range(20)]
+    acc = terms[0]
+    for k in range(1, 20):
+        acc = acc + terms[k] * terms[20 - k]
+    return acc
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(128, 128), (256, 200)])
+def test_wide_program_with_fused_binning(device, dtype, shape):
+    """A user component that keeps 20 values live compiles to a program wider
+    than 16 registers; run on the scratch-register kernel, its outputs are
+    bit-exact to elementwise numpy and its fused binning (small and large
+    detector) counts exactly the pixels of those outputs."""
+    from temgymcore_amd.run import plan_run_to_image, run_to_image
+
+    Mixer = _wide_component()
+    rng = np.random.default_rng(19)
+    n = 200_000
+    x, y = (rng.uniform(-0.5, 0.5, n).astype(dtype) for _ in range(2))
+    dx, dy = (rng.uniform(-0.05, 0.05, n).astype(dtype) for _ in range(2))
+    ray = Ray(x=_dev(x, device), y=_dev(y, device), dx=_dev(dx, device), dy=_dev(dy, device), z=0.0, pathlength=0.0)
+    det = Detector(z=0.5, pixel_size=(0.02, 0.02), shape=shape)
+    model = [Mixer(z=0.0), det]
+    plan = plan_run_to_image(ray, model)
+    regs = [max(p.instr[i].dst, p.instr[i].a, p.instr[i].b) for p, _, _ in plan.programs for i in range(p.n_instr)]
+    assert max(regs) >= 16
+    out, img = run_to_image(ray, model)
+    torch.cuda.synchronize()
+    acc = _mixer_ref(x, dx, dtype)
+    dx1, dy1 = dx + acc * dtype(0.25), dy - acc * dtype(0.125)
+    np.testing.assert_array_equal(_host(out.dx), dx1)
+    np.testing.assert_array_equal(_host(out.dy), dy1)
+    xr, yr = x + dx1 * dtype(0.5), y + dy1 * dtype(0.5)
+    np.testing.assert_array_equal(_host(out.x), xr)
+    Tinv = O.metres_to_pixels_matrix((0.0, 0.0), (0.02, 0.02), shape)
+    py, px = O.metres_to_pixels(xr, yr, Tinv)
+    ref = O.into_image(py, px, shape)
+    assert ref.sum() > 0.3 * n
+    np.testing.assert_array_equal(img.cpu().numpy().astype(np.int64), ref)
