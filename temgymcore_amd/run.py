@@ -70,15 +70,38 @@ def _trace_model(ray, components, propagator):
     return ray
 
 
-def run_to_end(ray: Ray, components: Sequence, propagator: BasePropagator = FreeSpaceParaxial()) -> Ray:
-    """Propagate `ray` through all components; one kernel launch for tensor rays."""
+def _model_fn(ray, components, propagator, composed: bool):
+    """The per-ray function of the model: the elements step by step, or
+    (composed=True, affine models) their composed affine-plus-pathlength form."""
+    if not composed:
+        return lambda r: _trace_model(r, components, propagator)
+    from .compose import affine_form, apply_affine
+
+    z = ray.z
+    if isinstance(z, torch.Tensor):
+        if z.numel() != 1:
+            raise ValueError("composed=True needs one input z for all rays")
+        z = z.item()
+    form = affine_form(components, float(z), propagator)
+    return lambda r: apply_affine(form, r)
+
+
+def run_to_end(ray: Ray, components: Sequence, propagator: BasePropagator = FreeSpaceParaxial(), *,
+               composed: bool = False) -> Ray:
+    """Propagate `ray` through all components; one kernel launch for tensor rays.
+
+    composed=True evaluates an affine model through its composed
+    affine-plus-pathlength form (compose.affine_form): one short program
+    whatever the model's length, rounding differently from the stepwise
+    evaluation (about 1e-15 relative in float64)."""
+    fn = _model_fn(ray, components, propagator, composed)
     if _program.is_device_ray(ray):
-        return _program.apply_fn(lambda r: _trace_model(r, components, propagator), ray)
-    return _trace_model(ray, components, propagator)
+        return _program.apply_fn(fn, ray)
+    return fn(ray)
 
 
 def plan_run_to_image(ray: Ray, components: Sequence, acc=None, grid=None,
-                      propagator: BasePropagator = FreeSpaceParaxial()) -> "_program.Plan":
+                      propagator: BasePropagator = FreeSpaceParaxial(), *, composed: bool = False) -> "_program.Plan":
     """Compile ``run_to_image`` once; ``plan.launch()`` re-runs it (the caller
     zeroes ``plan.acc`` between launches if counts should not accumulate)."""
     from .grid import Grid, _prepare_acc, make_bin_spec
@@ -92,7 +115,7 @@ def plan_run_to_image(ray: Ray, components: Sequence, acc=None, grid=None,
     if not _program.is_device_ray(ray):
         ray = ray.to_vector()
     plan = _program.plan_fn(
-        lambda r: _trace_model(r, components, propagator), ray,
+        _model_fn(ray, components, propagator, composed), ray,
         bin_request_fn=lambda r: (spec, r.x, r.y),
     )
     plan.acc = acc
@@ -100,20 +123,20 @@ def plan_run_to_image(ray: Ray, components: Sequence, acc=None, grid=None,
 
 
 def run_to_image(ray: Ray, components: Sequence, acc=None, grid=None,
-                 propagator: BasePropagator = FreeSpaceParaxial()):
+                 propagator: BasePropagator = FreeSpaceParaxial(), *, composed: bool = False):
     """``run_to_end`` + ``grid.into_image`` fused into one launch.
 
     `grid` defaults to the last component (it must be a Grid, e.g. Detector).
     Returns (ray_out, acc).
     """
-    plan = plan_run_to_image(ray, components, acc, grid, propagator)
+    plan = plan_run_to_image(ray, components, acc, grid, propagator, composed=composed)
     return plan.launch(), plan.acc
 
 
-def plan_run_to_end(ray: Ray, components: Sequence,
-                    propagator: BasePropagator = FreeSpaceParaxial()) -> "_program.Plan":
+def plan_run_to_end(ray: Ray, components: Sequence, propagator: BasePropagator = FreeSpaceParaxial(), *,
+                    composed: bool = False) -> "_program.Plan":
     """Compile ``run_to_end`` for a vectorised ray once; ``plan.launch()`` runs it."""
-    return _program.plan_fn(lambda r: _trace_model(r, components, propagator), ray)
+    return _program.plan_fn(_model_fn(ray, components, propagator, composed), ray)
 
 
 def solve_model(ray: Ray, model: Sequence, propagator: BasePropagator = FreeSpaceParaxial()):

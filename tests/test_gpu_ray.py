@@ -273,3 +273,39 @@ def test_wide_program_with_fused_binning(device, dtype, shape):
     ref = O.into_image(py, px, shape)
     assert ref.sum() > 0.3 * n
     np.testing.assert_array_equal(img.cpu().numpy().astype(np.int64), ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_composed_form_long_model(device, dtype):
+    """A 25-element affine model composed into one affine-plus-pathlength
+    program (the stepwise model splits into several): within 1e-12 (f64) /
+    2e-6 (f32) of the size of the outputs of the oracle's stepwise run, and
+    its fused histogram counts exactly the pixels of its own positions."""
+    import sys
+    import os
+
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_host_compose import _affine_model
+
+    rng = np.random.default_rng(73)
+    err = tuple(float(v) for v in rng.normal(0, 1e-3, 12))
+    model, omodel = _affine_model(err, 4)
+    n = 1 << 20
+    x, y = (rng.uniform(-0.05, 0.05, n).astype(dtype) for _ in range(2))
+    dx, dy = (rng.uniform(-0.02, 0.02, n).astype(dtype) for _ in range(2))
+    ray = Ray(x=_dev(x, device), y=_dev(y, device), dx=_dev(dx, device), dy=_dev(dy, device), z=0.0, pathlength=0.0)
+    plan = plan_run_to_image(ray, model, composed=True)
+    assert len(plan.programs) == 1
+    stepwise = plan_run_to_image(ray, model)
+    assert len(stepwise.programs) > 1
+    out = plan.launch()
+    img = plan.acc
+    torch.cuda.synchronize()
+    ref = O.run_to_end(O.make_ray(*(a.astype(np.float64) for a in (x, y, dx, dy)), 0.0, 0.0), omodel)
+    tol = 1e-12 if dtype == np.float64 else 2e-6
+    for k in ("x", "y", "dx", "dy", "pathlength"):
+        got = _host(getattr(out, k)).astype(np.float64)
+        assert np.abs(got - ref[k]).max() <= tol * (1 + np.abs(ref[k]).max()), k
+    Tinv = O.metres_to_pixels_matrix((0.0, 0.0), (0.01, 0.01), (64, 64))
+    py, px = O.metres_to_pixels(_host(out.x), _host(out.y), Tinv)
+    np.testing.assert_array_equal(img.cpu().numpy().astype(np.int64), O.into_image(py, px, (64, 64)))
