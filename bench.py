@@ -41,18 +41,23 @@ HBM_PEAK = 8.0e12  # B/s, MI355X
 SKIP_CPU = False
 
 
-def _profiled_traffic(fname):
-    """HBM bytes per launch of a kernel as measured by the calibrated
-    FETCH_SIZE + WRITE_SIZE PMC runs summarised in profiles/<fname>
-    (rocprofv3 cannot run inside this process); None if absent."""
+def _profiled(fname, key, cast=float):
+    """A `key: value` line of the PMC summary profiles/<fname> (rocprofv3
+    cannot run inside this process); None if absent."""
     try:
         with open(os.path.join(ROOT, "profiles", fname)) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return int(line.split(":", 1)[1])
+                if line.startswith(key + ":"):
+                    return cast(line.split(":", 1)[1])
     except OSError:
         pass
     return None
+
+
+def _profiled_traffic(fname):
+    """HBM bytes per launch of a kernel as measured by the calibrated
+    FETCH_SIZE + WRITE_SIZE PMC runs summarised in profiles/<fname>."""
+    return _profiled(fname, "traffic_bytes_per_launch", int)
 
 
 def _events(n):
@@ -530,7 +535,10 @@ def bench_beamlet(steps: int, warmup: int):
                 "autotune": {"beamlet_splits": tuned_splits, "candidate_ms": tune_ms}},
         roofline={"bound": "valu", "achieved": achieved / 1e12, "peak": 157.3, "unit": "TFLOP/s",
                   "frac": achieved / 157.3e12, "traffic": None,
-                  "flops_per_eval": flops_per_eval},
+                  "flops_per_eval": flops_per_eval,
+                  "valu_busy_percent": _profiled("beamlet_sum_kernel-beamlet_131072_1024sq.md", "valu_busy_percent"),
+                  "valu_source": "profiles/beamlet_sum_kernel-beamlet_131072_1024sq.md (VALUBusy PMC): the sum is "
+                                 "bound by VALU issue; frac counts only the useful FP32 flops"},
         cpu_baseline=cpu,
     )
 
@@ -610,7 +618,10 @@ def bench_gauss_fit(steps: int, warmup: int):
         roofline={"bound": "valu", "achieved": achieved / 1e12, "peak": 157.3, "unit": "TFLOP/s",
                   "frac": achieved / 157.3e12, "traffic": None,
                   "flops_per_eval": {"forward": flops_fwd, "backward": flops_bwd},
-                  "kernels": "fit_forward_kernel + fit_moment_kernel (whole step timed)"},
+                  "kernels": "fit_forward_kernel + fit_moment_kernel (whole step timed)",
+                  "valu_busy_percent": _profiled("fit_forward_moment_kernels-gauss_fit_256.md", "valu_busy_percent"),
+                  "valu_source": "profiles/fit_forward_moment_kernels-gauss_fit_256.md (VALUBusy PMC of the "
+                                 "forward kernel)"},
         cpu_baseline=cpu,
     )
 
