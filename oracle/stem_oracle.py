@@ -147,6 +147,28 @@ def fd_grad(geom, p, targets, scan_idx=None, rel=1e-2, steps=None):
     return g
 
 
+def grad_abs_scale(geom, p, targets, scan_idx=None, rel=1e-2):
+    """sum over the loss terms of |d(r^2)/dp_k| = 2 sum |r| |dr/dp_k| per
+    parameter (r the detector residuals, dr/dp_k by central differences of
+    the residual fields).  Not a value under test: the scale of one f32
+    rounding per term of the gradient sums, which sets how far an f32
+    evaluation may sit from the float64 gradient when the sum cancels."""
+    p = np.asarray(p, dtype=np.float64)
+    t = np.asarray(targets, dtype=np.float64)
+    x, y = forward(geom, p, scan_idx)
+    rx, ry = x - t[..., 0], y - t[..., 1]
+    h = fd_steps(p, rel)
+    out = np.zeros_like(p)
+    for i in range(p.size):
+        a, b = p.copy(), p.copy()
+        a[i] += h[i]
+        b[i] -= h[i]
+        xa, ya = forward(geom, a, scan_idx)
+        xb, yb = forward(geom, b, scan_idx)
+        out[i] = 2.0 * (np.abs(rx * (xa - xb)).sum() + np.abs(ry * (ya - yb)).sum()) / (2.0 * h[i])
+    return out
+
+
 def backproject(geom, p, data4d, det_Tinv, img_Tinv, cam=0):
     """Back-projection of data4d [S, H, W] onto the sample image (float64):
     every ray whose detector pixel (round half even, int32 cast, in range)

@@ -349,6 +349,9 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 // Tile `it` (camera c = it / ntile, tile it % ntile) of scan position s:
 // kPerLane float4 per lane, lane-strided so each load instruction of a
 // wavefront covers 1 KiB of contiguous targets.  Streamed once: non-temporal.
+// Indices past the row's end are clamped to its last float4 (the tile sums
+// skip those pairs), so every load is issued unpredicated: predicated loads
+// made the compiler wait for all outstanding loads before the first use.
 __device__ __forceinline__ void load_tile(const f4* __restrict__ T4, int64_t S, int64_t s, int64_t npair,
                                           int64_t ntile, int64_t it, f4 (&dst)[kPerLane]) {
   const int64_t c = it / ntile, tile = it - c * ntile;
@@ -357,13 +360,16 @@ __device__ __forceinline__ void load_tile(const f4* __restrict__ T4, int64_t S, 
 #pragma unroll
   for (int k = 0; k < kPerLane; ++k) {
     const int64_t q = q0 + (int64_t)k * kThreads;
-    dst[k] = q < npair ? __builtin_nontemporal_load(T4 + row + q) : f4{0.f, 0.f, 0.f, 0.f};
+    dst[k] = __builtin_nontemporal_load(T4 + row + (q < npair ? q : npair - 1));
   }
 }
 
 // Per-scan-position gradient terms from the per-camera moments M[c][k]
 // (FP64 sums of the wavefront partials, waves in order); thread t < np
-// writes row t of the workspace column s.
+// writes row t of the workspace column s.  Every term is linear in the
+// moments with coefficients affine in the camera length L_c, so rows 0..17
+// need only the camera sums of the x and y residual totals, plain and
+// L-weighted; row 18 + c takes camera c alone and the last row sums |r|^2.
 __device__ __forceinline__ void fit_epilogue(const tg_stem_geometry& G, const double* __restrict__ P,
                                              const ScanPos& p, const double (&M)[TG_STEM_MAX_CAM][kMom], int64_t s,
                                              int64_t S, double* __restrict__ partials) {
@@ -371,22 +377,37 @@ __device__ __forceinline__ void fit_epilogue(const tg_stem_geometry& G, const do
   const int t = threadIdx.x;
   if (t >= np) return;
   const double* A = P;
-  const double d = G.defocus, kap = G.pivot;
-  const double vv[4] = {p.spx, p.spy, p.stx, p.sty};
+  const double* L = P + 18;
   double g = 0.0;
-  for (int c = 0; c < G.n_cam; ++c) {
-    const double L = P[18 + c];
+  if (t == np - 1) {
+    for (int c = 0; c < G.n_cam; ++c) g += M[c][6];
+  } else if (t >= 18) {
+    const int c = t - 18;
+    const double dbx_dl = row_dot(A, 2, p) + G.offsets[2];
+    const double dby_dl = row_dot(A, 3, p) + G.offsets[3];
+    g = 2.0 * (M[c][1] + dbx_dl * M[c][0] + M[c][5] + dby_dl * M[c][3]);
+  } else {
+    double sx = 0.0, sxl = 0.0, sy = 0.0, syl = 0.0;
+    for (int c = 0; c < G.n_cam; ++c) {
+      sx += M[c][0];
+      sxl += L[c] * M[c][0];
+      sy += M[c][3];
+      syl += L[c] * M[c][3];
+    }
     if (t < 16) {
+      // d/dA[r][k] of sum_c |r_c|^2: rows 0, 1 act on the offsets (x, y),
+      // rows 2, 3 on the slopes (scaled by L_c); column k picks v[k]
       const int r = t >> 2, k = t & 3;
-      const double m0 = (r == 0 || r == 2) ? M[c][0] : M[c][3];
-      const double scale = (r >= 2) ? L : 1.0;
-      g += 2.0 * vv[k] * scale * m0;
-    } else if (t == 16 || t == 17) {
-      // d(Bx, By) / d(spx, spy), with st = pivot * sp
-      const double dbx_dx = kap * d + (A[0] + A[2] * kap) + (A[8] + A[10] * kap) * L;
-      const double dbx_dy = (A[1] + A[3] * kap) + (A[9] + A[11] * kap) * L;
-      const double dby_dx = (A[4] + A[6] * kap) + (A[12] + A[14] * kap) * L;
-      const double dby_dy = kap * d + (A[5] + A[7] * kap) + (A[13] + A[15] * kap) * L;
+      const double vk = k == 0 ? p.spx : k == 1 ? p.spy : k == 2 ? p.stx : p.sty;
+      const double m = r == 0 ? sx : r == 1 ? sy : r == 2 ? sxl : syl;
+      g = 2.0 * vk * m;
+    } else {
+      // d(Bx, By) / d(spx, spy) = a + b L_c, with st = pivot * sp
+      const double d = G.defocus, kap = G.pivot;
+      const double ax_x = kap * d + (A[0] + A[2] * kap), bx_x = A[8] + A[10] * kap;
+      const double ax_y = A[1] + A[3] * kap, bx_y = A[9] + A[11] * kap;
+      const double ay_x = A[4] + A[6] * kap, by_x = A[12] + A[14] * kap;
+      const double ay_y = kap * d + (A[5] + A[7] * kap), by_y = A[13] + A[15] * kap;
       double dsx, dsy;
       if (t == 16) {  // step: sp is proportional to it
         dsx = p.spx / P[16];
@@ -395,13 +416,8 @@ __device__ __forceinline__ void fit_epilogue(const tg_stem_geometry& G, const do
         dsx = p.spy * (kPi / 180.0);
         dsy = -p.spx * (kPi / 180.0);
       }
-      g += 2.0 * ((dbx_dx * dsx + dbx_dy * dsy) * M[c][0] + (dby_dx * dsx + dby_dy * dsy) * M[c][3]);
-    } else if (t == 18 + c) {
-      const double dbx_dl = row_dot(A, 2, p) + G.offsets[2];
-      const double dby_dl = row_dot(A, 3, p) + G.offsets[3];
-      g += 2.0 * (M[c][1] + dbx_dl * M[c][0] + M[c][5] + dby_dl * M[c][3]);
-    } else if (t == np - 1) {
-      g += M[c][6];
+      g = 2.0 * ((ax_x * dsx + ax_y * dsy) * sx + (bx_x * dsx + bx_y * dsy) * sxl + (ay_x * dsx + ay_y * dsy) * sy +
+                 (by_x * dsx + by_y * dsy) * syl);
     }
   }
   partials[(int64_t)t * S + s] = g;

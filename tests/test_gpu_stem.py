@@ -4,7 +4,9 @@ Tolerances (stated per the north-star precision policy):
   detector coordinates: |d| <= 1e-6 * max(|ref|, 1e-6 m)   (f32 output)
   sample values:        |d| <= 2e-5 (values in [-pi, pi], f32 bilinear weights)
   fit gradient:         |d| <= 2e-4 * max|g| per block of parameters, vs central
-                        FD of the float64 loss on the same f32 targets
+                        FD of the float64 loss on the same f32 targets; on
+                        larger detectors plus 2^-24 * sum|d(r^2)/dp| (one f32
+                        rounding of every term, oracle.stem_oracle.grad_abs_scale)
   loss:                 relative 1e-5
 """
 import dataclasses
@@ -191,6 +193,48 @@ def test_fit_launch_variants_are_bitwise_equal(device):
         autotune.clear()
     assert torch.equal(outs[0][0], outs[1][0])
     assert torch.equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("n_side, scan_shape", [
+    (128, (9, 7)),    # fixed-column tile form, four full tiles per camera length
+    (72, (5, 6)),     # stepped form: 36 half-columns, second tile partial
+    (64, (37, 41)),   # fixed-column form, 1517 positions: several per workgroup
+])
+def test_fit_tile_forms_and_position_sharing(device, n_side, scan_shape):
+    """Both forms of the tile sums and the sharing of scan positions among the
+    scan variant's workgroups: the two launch variants agree bitwise on every
+    per-position row, and sampled rows match the FD oracle."""
+    from temgymcore_amd import autotune
+
+    print(f"[stem fit tiles n_side={n_side} scan={scan_shape}]", flush=True)
+    model, _ = stem.reference_workload(n_cam=2, scan_shape=scan_shape, n_side=n_side)
+    params = stem.params_tensor(model, device)
+    targets = _targets(model, device)
+    ws = stem.STEMFitWorkspace(model, device)
+    key = (model.n_scan, model.n_rays, model.n_cam)
+    rows = []
+    try:
+        for variant in (autotune.STEM_FIT_SCAN, autotune.STEM_FIT_CAMERA):
+            autotune.set_choice(autotune.STEM_FIT, key, variant)
+            ws.run(params, targets)
+            rows.append(ws.partials.clone())
+    finally:
+        autotune.clear()
+    assert torch.equal(rows[0], rows[1])
+    r = rows[0].cpu().numpy().T  # [S, np]
+    pick = np.random.default_rng(1).choice(model.n_scan, 3, replace=False)
+    t = targets.cpu().numpy()[:, pick]
+    ref_g = SO.fd_grad(_geom(model), model.params(), t, scan_idx=pick)
+    ref_loss = SO.loss(_geom(model), model.params(), t, scan_idx=pick)
+    # f32 floor: one f32 rounding of every term of the sums (the residuals
+    # cancel to ~1e-5 of their absolute sum on these detectors)
+    floor = 2.0 ** -24 * SO.grad_abs_scale(_geom(model), model.params(), t, scan_idx=pick)
+    sub = r[pick].sum(axis=0)
+    assert sub[-1] == pytest.approx(ref_loss, rel=1e-5)
+    for blk in (slice(0, 16), slice(16, 17), slice(17, 18), slice(18, 20)):
+        bar = 2e-4 * np.abs(ref_g[blk]).max() + floor[blk]
+        err = np.abs(sub[blk] - ref_g[blk])
+        assert np.all(err <= bar), (blk, err / bar)
 
 
 @pytest.mark.parametrize("scan_shape", [(12, 10), (65, 64), (1, 1)])

@@ -160,3 +160,26 @@ def test_backproject_weights_sum_to_ray_count():
     np.testing.assert_allclose(B, Wt, rtol=1e-15, atol=0)
     n_rays = 30 * 64
     assert abs(Wt.sum() - n_rays) < 1e-9 * n_rays
+
+
+def test_grad_abs_scale_bounds_the_gradient_and_is_tight_without_cancellation():
+    """|sum_j d(r_j^2)/dp| <= sum_j |d(r_j^2)/dp| for every parameter, with
+    equality when every term has one sign: a single scan position whose
+    x residuals are all +1e-9 m and y residuals 0 makes d(r^2)/dA[0][0] =
+    2 * 1e-9 * spx for every ray."""
+    rng = np.random.default_rng(4)
+    p = _params(rng)
+    t = np.stack(S.forward(GEOM, _params(rng)), -1)
+    g = S.fd_grad(GEOM, p, t)
+    scale = S.grad_abs_scale(GEOM, p, t)
+    assert np.all(scale >= np.abs(g) * (1 - 1e-9))
+    assert np.any(scale > 2 * np.abs(g))  # the terms of some sums cancel
+    idx = [7]
+    x, y = S.forward(GEOM, p, idx)
+    one = np.stack([x - 1e-9, y], -1)
+    g1 = S.fd_grad(GEOM, p, one, scan_idx=idx)
+    s1 = S.grad_abs_scale(GEOM, p, one, scan_idx=idx)
+    assert s1[0] == pytest.approx(abs(g1[0]), rel=1e-6)
+    spx, _ = S.scan_positions(GEOM["scan_shape"], p[16], p[17])
+    n = len(p[18:]) * GEOM["n_side"] ** 2
+    assert s1[0] == pytest.approx(2e-9 * abs(spx[7]) * n, rel=1e-6)
