@@ -14,7 +14,9 @@ Pinning (tests/test_oracle_stem.py): descanner4 == ray_oracle.descanner when
 the tilt columns are zero; scan positions match the reference's rotated
 step-vector closed form; FD gradients converge in h and agree with the exact
 derivative for parameters that enter linearly; bilinear lookup reproduces
-linear images exactly and pixel values at pixel centres.
+linear images exactly and pixel values at pixel centres; grad_abs_scale
+bounds |fd_grad| and equals it (and its closed form 2 r spx N) when every
+term has one sign.
 """
 from __future__ import annotations
 
