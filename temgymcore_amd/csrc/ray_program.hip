@@ -33,6 +33,8 @@ constexpr int kThreads = 256;
 static_assert(sizeof(tg_program) + sizeof(tg_bin_spec) <= 4096, "ray program exceeds the kernel-argument limit");
 constexpr int64_t kLdsBins = 32768;
 constexpr int64_t kMaxRaysPerGroup = 65535;
+// input slots whose loads are issued together (one memory round trip)
+constexpr int kSlotGroup = 4;
 
 template <typename T>
 __device__ __forceinline__ T imm_of(const tg_instr& in);
@@ -156,21 +158,37 @@ __global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, con
       idx[u] = base + (int64_t)u * kThreads + threadIdx.x;
       ok[u] = idx[u] < n;
     }
-    for (int s = 0; s < P.n_slot; ++s) {
-      T v[R];
-      const bool narrow = sizeof(T) == 8 && P.slot[s].dtype == TG_F32;
+    // Inputs, kSlotGroup slots at a time: every load of a group is issued
+    // before the first is consumed, so a group costs one memory round trip.
+    // The loads are unpredicated (indices of rays past n are clamped to ray
+    // n - 1, whose values are replaced by 0 on entering the register file;
+    // a group's unused tail re-reads the last slot): predicated loads made
+    // the compiler wait for each slot's loads before issuing the next.
+    for (int s0 = 0; s0 < P.n_slot; s0 += kSlotGroup) {
+      T v[kSlotGroup][R];
 #pragma unroll
-      for (int u = 0; u < R; ++u) {
-        int64_t off = idx[u];
-        if constexpr (!CONTIG) off = slot_offset(idx[u], P.shape, P.slot[s].stride);
-        if (narrow)
-          v[u] = ok[u] ? (T)((const float*)P.slot[s].ptr)[off] : T(0);
-        else
-          v[u] = ok[u] ? ((const T*)P.slot[s].ptr)[off] : T(0);
+      for (int g = 0; g < kSlotGroup; ++g) {
+        const int s = min(s0 + g, P.n_slot - 1);
+        const bool narrow = sizeof(T) == 8 && P.slot[s].dtype == TG_F32;
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+          const int64_t i = ok[u] ? idx[u] : n - 1;
+          int64_t off = i;
+          if constexpr (!CONTIG) off = slot_offset(i, P.shape, P.slot[s].stride);
+          if (narrow)
+            v[g][u] = (T)((const float*)P.slot[s].ptr)[off];
+          else
+            v[g][u] = ((const T*)P.slot[s].ptr)[off];
+        }
       }
-      const int dst = P.slot[s].reg;
+      // unconditional writes (a tail entry writes the last slot's value to
+      // its register again), so no load is sunk into a conditional block
 #pragma unroll
-      for (int u = 0; u < R; ++u) r[u].set(dst, v[u]);
+      for (int g = 0; g < kSlotGroup; ++g) {
+        const int dst = P.slot[min(s0 + g, P.n_slot - 1)].reg;
+#pragma unroll
+        for (int u = 0; u < R; ++u) r[u].set(dst, ok[u] ? v[g][u] : T(0));
+      }
     }
     for (int i = 0; i < P.n_instr; ++i) {
       const tg_instr in = P.instr[i];
