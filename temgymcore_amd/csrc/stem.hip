@@ -239,7 +239,31 @@ __global__ __launch_bounds__(kThreads) void stem_forward_kernel(const tg_stem_ge
 // (defocus x semi-convergence), so each workgroup accumulates into a
 // kBpTile^2 LDS tile centred on it and flushes that once; contributions
 // outside the tile go to global memory directly.  FP64 accumulation.
+//
+// Each thread takes a kBpBlock x kBpBlock block of adjacent rays, whose
+// sample positions span well under a pixel on the usual geometries: the
+// block's bilinear deposits then fall in a 3 x 3 pixel window anchored at
+// the floor of the block's smallest coordinates, accumulated in registers and
+// added to the tile once (9 LDS atomics per 16 rays instead of 64; same-
+// address LDS atomics from the lanes of a wavefront serialise, and bound the
+// one-ray-at-a-time form).  A ray whose cell falls outside the window (wide
+// footprints) deposits directly, as before.
 constexpr int kBpTile = 32;
+constexpr int kBpBlock = 4;                        // rays per side of one thread's block
+constexpr int kBpRayTile = 16 * kBpBlock;          // rays per side of one workgroup (16 x 16 blocks)
+static_assert(kThreads == 256, "one workgroup = 16 x 16 ray blocks");
+
+// Destination of a deposit to sample pixel (yy, xx): its index in the LDS
+// tile, kBpGlobal for global memory, kBpSkip outside the image.  (The atomics
+// stay at the call sites on the __shared__ arrays themselves, so that they
+// compile to LDS atomics rather than flat ones.)
+constexpr int kBpGlobal = -1, kBpSkip = -2;
+
+__device__ __forceinline__ int bp_slot(const tg_stem_geometry& G, int oy, int ox, int yy, int xx) {
+  if (yy < 0 || yy >= G.img_h || xx < 0 || xx >= G.img_w) return kBpSkip;
+  const int ty_ = yy - oy, tx_ = xx - ox;
+  return ty_ >= 0 && ty_ < kBpTile && tx_ >= 0 && tx_ < kBpTile ? ty_ * kBpTile + tx_ : kBpGlobal;
+}
 
 __global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_stem_geometry G,
                                                                     const double* __restrict__ P,
@@ -251,8 +275,8 @@ __global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_ste
   __shared__ double t_w[kBpTile * kBpTile];
   __shared__ float sh_a, sh_bx, sh_by;
   const int64_t s = blockIdx.y;
-  const int64_t J = (int64_t)G.n_side * G.n_side;
-  for (int k = threadIdx.x; k < G.n_side; k += kThreads) th[k] = slope_of(G, k);
+  const int n = G.n_side;
+  for (int k = threadIdx.x; k < n; k += kThreads) th[k] = slope_of(G, k);
   for (int k = threadIdx.x; k < kBpTile * kBpTile; k += kThreads) {
     t_img[k] = 0.0;
     t_w[k] = 0.0;
@@ -273,51 +297,102 @@ __global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_ste
   const int oy = (int)fmin(fmax(oyd, -4.0 * kBpTile), (double)G.img_h + 4.0 * kBpTile);
   const int ox = (int)fmin(fmax(oxd, -4.0 * kBpTile), (double)G.img_w + 4.0 * kBpTile);
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kThreads * kFwdRays;
-  const int n = G.n_side, drow = kThreads / n, dcol = kThreads % n;
-  const int j0 = (int)(base + threadIdx.x);
-  int iy = j0 / n, ix = j0 - (j0 / n) * n;
-#pragma unroll 4
-  for (int u = 0; u < kFwdRays; ++u, ix += dcol, iy += drow) {
-    if (ix >= n) {
-      ix -= n;
-      ++iy;
+  // this thread's block of rays: rows iy0.., columns ix0.. (clipped to n)
+  const int ntx = (n + kBpRayTile - 1) / kBpRayTile;
+  const int iy0 = (int)(blockIdx.x / ntx) * kBpRayTile + (int)(threadIdx.x / 16) * kBpBlock;
+  const int ix0 = (int)(blockIdx.x % ntx) * kBpRayTile + (int)(threadIdx.x % 16) * kBpBlock;
+  if (iy0 < n && ix0 < n) {
+    // window anchor: floor of the smallest sample coordinates over the
+    // block's corners (the position is affine in the slopes)
+    const int iy1 = min(iy0 + kBpBlock - 1, n - 1), ix1 = min(ix0 + kBpBlock - 1, n - 1);
+    double cy0, cx0, cy1, cx1, cy2, cx2, cy3, cx3;
+    sample_pixel(smap, th[ix0], th[iy0], cy0, cx0);
+    sample_pixel(smap, th[ix1], th[iy0], cy1, cx1);
+    sample_pixel(smap, th[ix0], th[iy1], cy2, cx2);
+    sample_pixel(smap, th[ix1], th[iy1], cy3, cx3);
+    const double wy0 = floor(fmin(fmin(cy0, cy1), fmin(cy2, cy3)));
+    const double wx0 = floor(fmin(fmin(cx0, cx1), fmin(cx2, cx3)));
+    // anchors far outside the image: every ray deposits directly (and skips)
+    const bool anchored = wy0 > -4.0 && wy0 < (double)G.img_h + 4.0 && wx0 > -4.0 && wx0 < (double)G.img_w + 4.0;
+    const int Y = anchored ? (int)wy0 : 0, X = anchored ? (int)wx0 : 0;
+    double acc[9], accw[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      acc[k] = 0.0;
+      accw[k] = 0.0;
     }
-    const int64_t j = base + (int64_t)u * kThreads + threadIdx.x;
-    if (j >= J) continue;
-    const float tx = th[ix], ty = th[iy];
-    // detector pixel, as in stem_forward_kernel
-    const float xd = fmaf(tx, sh_a, sh_bx), yd = fmaf(ty, sh_a, sh_by);
-    const double dfy = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[0], (double)yd), __dmul_rn(G.det_m[1], (double)xd)),
-                                 G.det_m[2]);
-    const double dfx = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[3], (double)yd), __dmul_rn(G.det_m[4], (double)xd)),
-                                 G.det_m[5]);
-    const int32_t py = tg::xla_f64_to_i32(rint(dfy)), px = tg::xla_f64_to_i32(rint(dfx));
-    if (py < 0 || py >= G.det_h || px < 0 || px >= G.det_w) continue;
-    const double d = (double)data[(s * G.det_h + py) * G.det_w + px];
-    // bilinear weights at the sample-plane position, as in bilinear_tiled()
-    double fy, fx;
-    sample_pixel(smap, tx, ty, fy, fx);
-    const double y0d = floor(fy), x0d = floor(fx);
-    if (!(y0d >= -1.0 && y0d < (double)G.img_h && x0d >= -1.0 && x0d < (double)G.img_w)) continue;
-    const int y0 = (int)y0d, x0 = (int)x0d;
-    const float wy = (float)(fy - y0d), wx = (float)(fx - x0d);
+    for (int r = 0; r < kBpBlock; ++r) {
+      const int iy = iy0 + r;
+      if (iy >= n) break;
+      const float ty = th[iy];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
+      for (int c = 0; c < kBpBlock; ++c) {
+        const int ix = ix0 + c;
+        if (ix >= n) continue;
+        const float tx = th[ix];
+        // detector pixel, as in stem_forward_kernel
+        const float xd = fmaf(tx, sh_a, sh_bx), yd = fmaf(ty, sh_a, sh_by);
+        const double dfy = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[0], (double)yd), __dmul_rn(G.det_m[1], (double)xd)),
+                                     G.det_m[2]);
+        const double dfx = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[3], (double)yd), __dmul_rn(G.det_m[4], (double)xd)),
+                                     G.det_m[5]);
+        const int32_t py = tg::xla_f64_to_i32(rint(dfy)), px = tg::xla_f64_to_i32(rint(dfx));
+        if (py < 0 || py >= G.det_h || px < 0 || px >= G.det_w) continue;
+        const double d = (double)data[(s * G.det_h + py) * G.det_w + px];
+        // bilinear weights at the sample-plane position, as in bilinear_tiled()
+        double fy, fx;
+        sample_pixel(smap, tx, ty, fy, fx);
+        const double y0d = floor(fy), x0d = floor(fx);
+        if (!(y0d >= -1.0 && y0d < (double)G.img_h && x0d >= -1.0 && x0d < (double)G.img_w)) continue;
+        const int y0 = (int)y0d, x0 = (int)x0d;
+        const float wy = (float)(fy - y0d), wx = (float)(fx - x0d);
+        const int dy = y0 - Y, dx = x0 - X;
+        if (anchored && (unsigned)dy <= 1u && (unsigned)dx <= 1u) {
+          // separable weights placed in the 3 x 3 window: row i of the
+          // window carries (1 - wy) at dy and wy at dy + 1, likewise columns
+          const float ry[3] = {dy == 0 ? 1.0f - wy : 0.0f, dy == 0 ? wy : 1.0f - wy, dy == 0 ? 0.0f : wy};
+          const float rx[3] = {dx == 0 ? 1.0f - wx : 0.0f, dx == 0 ? wx : 1.0f - wx, dx == 0 ? 0.0f : wx};
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const int yy = y0 + a, xx = x0 + b;
-        if (yy < 0 || yy >= G.img_h || xx < 0 || xx >= G.img_w) continue;
-        const double w = (double)((a ? wy : 1.0f - wy) * (b ? wx : 1.0f - wx));
-        const int ty_ = yy - oy, tx_ = xx - ox;
-        if (ty_ >= 0 && ty_ < kBpTile && tx_ >= 0 && tx_ < kBpTile) {
-          atomicAdd(&t_img[ty_ * kBpTile + tx_], w * d);
-          if (weight != nullptr) atomicAdd(&t_w[ty_ * kBpTile + tx_], w);
+          for (int i = 0; i < 3; ++i) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              const double w = (double)(ry[i] * rx[k]);
+              acc[3 * i + k] += w * d;
+              accw[3 * i + k] += w;
+            }
+          }
         } else {
-          const int64_t q = (int64_t)yy * G.img_w + xx;
-          unsafeAtomicAdd(image + q, w * d);
-          if (weight != nullptr) unsafeAtomicAdd(weight + q, w);
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+              const double w = (double)((a ? wy : 1.0f - wy) * (b ? wx : 1.0f - wx));
+              const int q = bp_slot(G, oy, ox, y0 + a, x0 + b);
+              if (q >= 0) {
+                atomicAdd(&t_img[q], w * d);
+                if (weight != nullptr) atomicAdd(&t_w[q], w);
+              } else if (q == kBpGlobal) {
+                const int64_t gq = (int64_t)(y0 + a) * G.img_w + (x0 + b);
+                unsafeAtomicAdd(image + gq, w * d);
+                if (weight != nullptr) unsafeAtomicAdd(weight + gq, w);
+              }
+            }
+          }
         }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      if (accw[k] == 0.0) continue;  // no ray of the block weighted this pixel
+      const int yy = Y + k / 3, xx = X + k % 3;
+      const int q = bp_slot(G, oy, ox, yy, xx);
+      if (q >= 0) {
+        atomicAdd(&t_img[q], acc[k]);
+        if (weight != nullptr) atomicAdd(&t_w[q], accw[k]);
+      } else if (q == kBpGlobal) {
+        const int64_t gq = (int64_t)yy * G.img_w + xx;
+        unsafeAtomicAdd(image + gq, acc[k]);
+        if (weight != nullptr) unsafeAtomicAdd(weight + gq, accw[k]);
       }
     }
   }
@@ -688,8 +763,8 @@ extern "C" int tg_stem_backproject(const tg_stem_geometry* g, const double* para
   TG_REQUIRE(cam >= 0 && cam < g->n_cam, "tg_stem_backproject: camera %d out of range", cam);
   TG_REQUIRE(g->img_h > 0 && g->img_w > 0 && g->det_h > 0 && g->det_w > 0,
              "tg_stem_backproject: bad sample or detector shape");
-  const int64_t J = (int64_t)g->n_side * g->n_side;
-  dim3 grid((unsigned)tg::cdiv(J, (int64_t)kThreads * kFwdRays), (unsigned)((int64_t)g->scan_h * g->scan_w));
+  const int64_t ntile = tg::cdiv((int64_t)g->n_side, (int64_t)kBpRayTile);
+  dim3 grid((unsigned)(ntile * ntile), (unsigned)((int64_t)g->scan_h * g->scan_w));
   hipLaunchKernelGGL(stem_backproject_kernel, grid, dim3(kThreads), 0, (hipStream_t)stream, *g, params, data4d, cam,
                      image, weight);
   TG_LAUNCH_CHECK("stem_backproject_kernel launch");

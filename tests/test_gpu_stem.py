@@ -309,14 +309,16 @@ def _bp_oracle(model, D_host, cam=0):
     return SO.backproject(geom, model.params(), D_host, det_Tinv, img_Tinv, cam)
 
 
-@pytest.mark.parametrize("defocus", [200e-9, 2e-6])
-def test_backproject_vs_oracle(device, defocus):
+@pytest.mark.parametrize("defocus, n_side", [(200e-9, 16), (2e-6, 16), (200e-9, 74)])
+def test_backproject_vs_oracle(device, defocus, n_side):
     """Constant detector images per scan position (so the detector binning of
     a ray on a pixel edge cannot matter) back-project to the oracle's sums.
     At 2 um defocus the footprints exceed the workgroup's LDS tile and the
-    global-memory path carries the outer rays.  Tolerance: the kernel's
-    bilinear weights are f32 products (as in the forward), 1e-6 relative."""
-    model, _ = stem.reference_workload(n_cam=2, scan_shape=(8, 8), n_side=16, defocus=defocus, det_pixel=1e-5)
+    global-memory path carries the outer rays (and each thread's 4 x 4 ray
+    block spans more than its 3 x 3 pixel window); 74 rays per side leave
+    partial blocks and workgroups.  Tolerance: the kernel's bilinear weights
+    are f32 products (as in the forward), 1e-6 relative."""
+    model, _ = stem.reference_workload(n_cam=2, scan_shape=(8, 8), n_side=n_side, defocus=defocus, det_pixel=1e-5)
     rng = np.random.default_rng(12)
     c = rng.uniform(0.5, 2.0, model.n_scan)
     D = np.broadcast_to(c[:, None, None], (model.n_scan,) + tuple(model.det_shape)).astype(np.float32)
