@@ -151,17 +151,33 @@ __global__ __launch_bounds__(kThreads) void stem_forward_kernel(const tg_stem_ge
   // image only send every lookup to global memory)
   const int oy = (int)fmin(fmax(floor(smap.fy0) - kImgTile / 2, -4.0 * kImgTile), (double)G.img_h + 4.0 * kImgTile);
   const int ox = (int)fmin(fmax(floor(smap.fx0) - kImgTile / 2, -4.0 * kImgTile), (double)G.img_w + 4.0 * kImgTile);
+  // camera length of this thread's camera constants, loaded before the tile
+  // so that both arrive in one round trip
+  const double Lc = P[18 + min((int)threadIdx.x, G.n_cam - 1)];
   if (img != nullptr) {
-    for (int k = threadIdx.x; k < kImgTile * kImgTile; k += kThreads) {
+    // all of a thread's tile loads are issued before the first is used
+    // (clamped indices, unpredicated; pixels outside the image become 0):
+    // predicated loads in a loop made each one wait for the previous
+    constexpr int kPerThread = kImgTile * kImgTile / kThreads;
+    static_assert(kPerThread * kThreads == kImgTile * kImgTile, "tile loads per thread");
+    float tv[kPerThread];
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+      const int k = (int)threadIdx.x + i * kThreads;
+      const int yy = min(max(oy + k / kImgTile, 0), G.img_h - 1), xx = min(max(ox + k % kImgTile, 0), G.img_w - 1);
+      tv[i] = img[(int64_t)yy * G.img_w + xx];
+    }
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+      const int k = (int)threadIdx.x + i * kThreads;
       const int yy = oy + k / kImgTile, xx = ox + k % kImgTile;
-      tile[k] = (yy >= 0 && yy < G.img_h && xx >= 0 && xx < G.img_w) ? img[(int64_t)yy * G.img_w + xx] : 0.0f;
+      tile[k] = (yy >= 0 && yy < G.img_h && xx >= 0 && xx < G.img_w) ? tv[i] : 0.0f;
     }
   }
   if (threadIdx.x < G.n_cam) {
-    const double L = P[18 + threadIdx.x];
     double bx, by;
-    offsets_at(G, P, p, L, bx, by);
-    sh_a[threadIdx.x] = (float)(G.defocus + L);
+    offsets_at(G, P, p, Lc, bx, by);
+    sh_a[threadIdx.x] = (float)(G.defocus + Lc);
     sh_bx[threadIdx.x] = (float)bx;
     sh_by[threadIdx.x] = (float)by;
   }
