@@ -33,8 +33,10 @@ constexpr int kThreads = 256;
 static_assert(sizeof(tg_program) + sizeof(tg_bin_spec) <= 4096, "ray program exceeds the kernel-argument limit");
 constexpr int64_t kLdsBins = 32768;
 constexpr int64_t kMaxRaysPerGroup = 65535;
-// input slots whose loads are issued together (one memory round trip)
-constexpr int kSlotGroup = 4;
+// input slots whose loads are issued together (one memory round trip);
+// strided slots carry 64-bit offsets per ray, so they go two at a time
+template <bool CONTIG>
+constexpr int kSlotGroup = CONTIG ? 4 : 2;
 
 template <typename T>
 __device__ __forceinline__ T imm_of(const tg_instr& in);
@@ -135,7 +137,7 @@ __device__ __forceinline__ void global_deposit(const tg_bin_spec& B, int64_t p, 
 // BIN: 0 = no binning, 1 = LDS-privatised integer histogram, 2 = global atomics.
 // NT: workgroup size (LDS-histogram launches use 1024 so that 16 wavefronts
 // share one private histogram and fewer flushes hit the same global bins).
-template <typename T, int R, bool CONTIG, int BIN, int NT, int NREG>
+template <typename T, int R, bool CONTIG, int BIN, int NT, int NREG, bool MIXED>
 __global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, const tg_bin_spec B) {
   constexpr int kThreads = NT;
   extern __shared__ int32_t lds_hist[];
@@ -164,27 +166,32 @@ __global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, con
     // n - 1, whose values are replaced by 0 on entering the register file;
     // a group's unused tail re-reads the last slot): predicated loads made
     // the compiler wait for each slot's loads before issuing the next.
-    for (int s0 = 0; s0 < P.n_slot; s0 += kSlotGroup) {
-      T v[kSlotGroup][R];
+    for (int s0 = 0; s0 < P.n_slot; s0 += kSlotGroup<CONTIG>) {
+      T v[kSlotGroup<CONTIG>][R];
 #pragma unroll
-      for (int g = 0; g < kSlotGroup; ++g) {
+      for (int g = 0; g < kSlotGroup<CONTIG>; ++g) {
         const int s = min(s0 + g, P.n_slot - 1);
-        const bool narrow = sizeof(T) == 8 && P.slot[s].dtype == TG_F32;
 #pragma unroll
         for (int u = 0; u < R; ++u) {
           const int64_t i = ok[u] ? idx[u] : n - 1;
           int64_t off = i;
           if constexpr (!CONTIG) off = slot_offset(i, P.shape, P.slot[s].stride);
-          if (narrow)
-            v[g][u] = (T)((const float*)P.slot[s].ptr)[off];
-          else
+          if constexpr (MIXED) {
+            // an f64 program with f32 slots: the per-slot width test is a
+            // branch, and its loads are waited for one at a time
+            if (P.slot[s].dtype == TG_F32)
+              v[g][u] = (T)((const float*)P.slot[s].ptr)[off];
+            else
+              v[g][u] = ((const T*)P.slot[s].ptr)[off];
+          } else {
             v[g][u] = ((const T*)P.slot[s].ptr)[off];
+          }
         }
       }
       // unconditional writes (a tail entry writes the last slot's value to
       // its register again), so no load is sunk into a conditional block
 #pragma unroll
-      for (int g = 0; g < kSlotGroup; ++g) {
+      for (int g = 0; g < kSlotGroup<CONTIG>; ++g) {
         const int dst = P.slot[min(s0 + g, P.n_slot - 1)].reg;
 #pragma unroll
         for (int u = 0; u < R; ++u) r[u].set(dst, ok[u] ? v[g][u] : T(0));
@@ -341,7 +348,7 @@ __global__ __launch_bounds__(NT) void ray_program_kernel(const tg_program P, con
   }
 }
 
-template <typename T, int R, bool CONTIG, int BIN, int NREG>
+template <typename T, int R, bool CONTIG, int BIN, int NREG, bool MIXED>
 int launch(const tg_program& P, const tg_bin_spec& B, hipStream_t stream) {
   constexpr int kThreads = (BIN == 1) ? 1024 : 256;
   const int64_t chunk = (int64_t)kThreads * R;
@@ -366,26 +373,37 @@ int launch(const tg_program& P, const tg_bin_spec& B, hipStream_t stream) {
     if (grid < min_grid) grid = min_grid;
   }
   const size_t lds = (BIN == 1) ? (size_t)((B.height * B.width + 1) / 2) * sizeof(int32_t) : 0;
-  hipLaunchKernelGGL((ray_program_kernel<T, R, CONTIG, BIN, kThreads, NREG>), dim3((unsigned)grid), dim3(kThreads),
+  hipLaunchKernelGGL((ray_program_kernel<T, R, CONTIG, BIN, kThreads, NREG, MIXED>), dim3((unsigned)grid), dim3(kThreads),
                      lds, stream, P, B);
   TG_LAUNCH_CHECK("ray_program_kernel launch");
   return TG_OK;
 }
 
-template <typename T, int R, int NREG>
-int dispatch(const tg_program& P, const tg_bin_spec& B, bool contig, int bin, hipStream_t s) {
+template <typename T, int R, int NREG, bool MIXED>
+int dispatch_width(const tg_program& P, const tg_bin_spec& B, bool contig, int bin, hipStream_t s) {
   // wide programs bin with global atomics (one kernel fewer per file width)
   if (NREG > 16 && bin == 1) bin = 2;
   if (contig) {
-    if (bin == 0) return launch<T, R, true, 0, NREG>(P, B, s);
+    if (bin == 0) return launch<T, R, true, 0, NREG, MIXED>(P, B, s);
     if constexpr (NREG == 16)
-      if (bin == 1) return launch<T, R, true, 1, NREG>(P, B, s);
-    return launch<T, R, true, 2, NREG>(P, B, s);
+      if (bin == 1) return launch<T, R, true, 1, NREG, MIXED>(P, B, s);
+    return launch<T, R, true, 2, NREG, MIXED>(P, B, s);
   }
-  if (bin == 0) return launch<T, R, false, 0, NREG>(P, B, s);
+  if (bin == 0) return launch<T, R, false, 0, NREG, MIXED>(P, B, s);
   if constexpr (NREG == 16)
-    if (bin == 1) return launch<T, R, false, 1, NREG>(P, B, s);
-  return launch<T, R, false, 2, NREG>(P, B, s);
+    if (bin == 1) return launch<T, R, false, 1, NREG, MIXED>(P, B, s);
+  return launch<T, R, false, 2, NREG, MIXED>(P, B, s);
+}
+
+// f64 programs whose slots are all f64 load them without a per-slot width
+// test; only programs that also read f32 slots take the mixed variant
+template <typename T, int R, int NREG>
+int dispatch(const tg_program& P, const tg_bin_spec& B, bool contig, int bin, hipStream_t s) {
+  if constexpr (sizeof(T) == 8) {
+    for (int k = 0; k < P.n_slot; ++k)
+      if (P.slot[k].dtype == TG_F32) return dispatch_width<T, R, NREG, true>(P, B, contig, bin, s);
+  }
+  return dispatch_width<T, R, NREG, false>(P, B, contig, bin, s);
 }
 
 // Highest register the program (and its binning) touches.
