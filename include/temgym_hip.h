@@ -31,7 +31,7 @@ extern "C" {
 const char* tg_last_error(void);
 
 /* ABI version; bumped whenever a struct layout below changes. */
-int32_t tg_abi_version(void);  /* currently 5 */
+int32_t tg_abi_version(void);  /* currently 6 */
 
 /* ======================================================================== *
  * Ray branch: per-ray register programs
@@ -203,8 +203,9 @@ int tg_stem_forward(const tg_stem_geometry* g, const double* params, const float
 /* Descan / geometry fit, forward and adjoint in one pass over the targets.
  *   targets  [n_cam, S, J, 2] f32 target detector coordinates
  *   workspace tg_stem_fit_workspace(g) bytes of f64: starts with the
- *            [18 + n_cam + 1, S] per-scan-position terms (one row per
- *            output entry), then [S, n_cam, 7] per-camera moments
+ *            [S, 18 + n_cam + 1] per-scan-position terms (one row per
+ *            scan position, one column per output entry), then
+ *            [S, n_cam, 7] per-camera moments
  *   out      [18 + n_cam + 1] f64: d loss / d params, then the loss
  * loss = sum over (c, s, j) of |detector(x, y) - target|^2.
  * Two launch variants give identical results: TG_STEM_FIT_SCAN (one
@@ -224,8 +225,8 @@ int tg_stem_backproject(const tg_stem_geometry* g, const double* params, const f
                         double* image, double* weight, void* stream);
 
 /* The launches of tg_stem_fit before its final reduction: fills the head of
- * the workspace with the [18 + n_cam + 1, S] per-scan-position terms (row t,
- * column s: scan position s's contribution to out[t]; the last row is its
+ * the workspace with the [S, 18 + n_cam + 1] per-scan-position terms (row s,
+ * column t: scan position s's contribution to out[t]; the last column is its
  * share of the loss, a residual map over the scan).  tg_stem_fit = this +
  * the fixed-order FP64 sums over s. */
 int tg_stem_fit_terms(const tg_stem_geometry* g, const double* params, const float* targets, double* workspace,

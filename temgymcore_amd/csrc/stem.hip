@@ -457,7 +457,9 @@ __device__ __forceinline__ void load_tile(const f4* __restrict__ T4, int64_t S, 
 
 // Per-scan-position gradient terms from the per-camera moments M[c][k]
 // (FP64 sums of the wavefront partials, waves in order); thread t < np
-// writes row t of the workspace column s.  Every term is linear in the
+// writes entry t of the workspace row s (the np terms of one scan position
+// are contiguous: one short coalesced store per workgroup, where a column
+// layout cost 27 scattered 8-byte stores and ~9 us per pass).  Every term is linear in the
 // moments with coefficients affine in the camera length L_c, so rows 0..17
 // need only the camera sums of the x and y residual totals, plain and
 // L-weighted; row 18 + c takes camera c alone and the last row sums |r|^2.
@@ -511,7 +513,7 @@ __device__ __forceinline__ void fit_epilogue(const tg_stem_geometry& G, const do
                  (by_x * dsx + by_y * dsy) * syl);
     }
   }
-  partials[(int64_t)t * S + s] = g;
+  partials[s * np + t] = g;
 }
 
 // Camera-length constants of the residual r = th a + b - t as double-float
@@ -618,9 +620,11 @@ __global__ __launch_bounds__(kThreads) void stem_fit_kernel(const tg_stem_geomet
   __shared__ float th[kMaxSide];
   __shared__ double wm[TG_STEM_MAX_CAM][kWaves][kMom];
   __shared__ double M[TG_STEM_MAX_CAM][kMom];
+  __shared__ double sp[18 + TG_STEM_MAX_CAM];  // the parameters, read by the set-up and the epilogue
   const int64_t S = (int64_t)G.scan_h * G.scan_w;
   const int64_t J = (int64_t)G.n_side * G.n_side;
   fill_slopes(G, th);
+  if (threadIdx.x < 18 + G.n_cam) sp[threadIdx.x] = P[threadIdx.x];
   __syncthreads();
   const int64_t npair = J >> 1;  // two rays per float4 of targets (n_side even)
   const int64_t ntile = (npair + kTile - 1) / kTile;
@@ -629,7 +633,7 @@ __global__ __launch_bounds__(kThreads) void stem_fit_kernel(const tg_stem_geomet
   f4 cur[kPerLane], nxt[kPerLane];
   const int64_t s = blockIdx.x;
   load_tile(T4, S, s, npair, ntile, 0, cur);
-  const ScanPos p = scan_position(G, P, s);
+  const ScanPos p = scan_position(G, sp, s);
   CamConst K = {};
   float m[kMom];
   for (int64_t it = 0; it < nit; ++it) {
@@ -637,7 +641,7 @@ __global__ __launch_bounds__(kThreads) void stem_fit_kernel(const tg_stem_geomet
     const int c = (int)(it / ntile);
     const int64_t tile = it - (int64_t)c * ntile;
     if (tile == 0) {
-      K = cam_const(G, P, p, c);
+      K = cam_const(G, sp, p, c);
 #pragma unroll
       for (int k = 0; k < kMom; ++k) m[k] = 0.f;
     }
@@ -655,7 +659,7 @@ __global__ __launch_bounds__(kThreads) void stem_fit_kernel(const tg_stem_geomet
     M[c][k] = acc;
   }
   __syncthreads();
-  fit_epilogue(G, P, p, M, s, S, partials);
+  fit_epilogue(G, sp, p, M, s, S, partials);
 }
 
 // Variant "camera": one workgroup per (scan position, camera length) --
@@ -714,8 +718,9 @@ __global__ __launch_bounds__(64) void stem_fit_epilogue(const tg_stem_geometry G
   fit_epilogue(G, P, p, M, s, S, partials);
 }
 
-// Fixed-order FP64 row sums of the [np, rows] workspace (one block per row,
-// coalesced reads, wavefront shuffles then one LDS step).
+// Fixed-order FP64 column sums of the [rows, cols] workspace (one block per
+// column, strided reads of the L2-resident terms, wavefront shuffles then
+// one LDS step).
 constexpr int kSumThreads = 1024;
 
 __device__ __forceinline__ double wave_sum_f64(double v) {
@@ -725,11 +730,11 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 }
 
 __global__ __launch_bounds__(kSumThreads) void row_sum_kernel(const double* __restrict__ partials, int64_t rows,
-                                                              double* __restrict__ out) {
+                                                              int64_t cols, double* __restrict__ out) {
   __shared__ double red[kSumThreads / 64];
-  const double* __restrict__ src = partials + (int64_t)blockIdx.x * rows;
+  const double* __restrict__ src = partials + blockIdx.x;  // column blockIdx.x of [rows, cols]
   double acc = 0.0;
-  for (int64_t r = threadIdx.x; r < rows; r += kSumThreads) acc += src[r];
+  for (int64_t r = threadIdx.x; r < rows; r += kSumThreads) acc += src[r * cols];
   acc = wave_sum_f64(acc);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
@@ -795,7 +800,7 @@ extern "C" int64_t tg_stem_fit_workspace(const tg_stem_geometry* g) {
 
 namespace {
 
-// The per-scan-position terms [np, S] at the head of the workspace (the
+// The per-scan-position terms [S, np] at the head of the workspace (the
 // launches of tg_stem_fit before its final reduction).
 int fit_terms(const tg_stem_geometry* g, const double* params, const float* targets, double* workspace,
               hipStream_t stream) {
@@ -803,7 +808,7 @@ int fit_terms(const tg_stem_geometry* g, const double* params, const float* targ
   TG_REQUIRE(((uintptr_t)targets & 15) == 0, "tg_stem_fit: targets must be 16-byte aligned");
   const int64_t S = (int64_t)g->scan_h * g->scan_w;
   const int np = 18 + g->n_cam + 1;
-  double* partials = workspace;                   // [np, S]
+  double* partials = workspace;                   // [S, np]
   double* cam_mom = workspace + (int64_t)np * S;  // [S, n_cam, 7]
   // launch variant: the autotuner's choice, else one workgroup per scan
   // position when that gives every CU several workgroups, and one per (scan
@@ -848,7 +853,7 @@ extern "C" int tg_stem_fit(const tg_stem_geometry* g, const double* params, cons
   const int64_t S = (int64_t)g->scan_h * g->scan_w;
   const int np = 18 + g->n_cam + 1;
   hipLaunchKernelGGL(row_sum_kernel, dim3((unsigned)np), dim3(kSumThreads), 0, (hipStream_t)stream, workspace, S,
-                     out);
+                     (int64_t)np, out);
   TG_LAUNCH_CHECK("row_sum_kernel launch");
   return TG_OK;
 }

@@ -287,8 +287,9 @@ class STEMFitWorkspace:
         self.np = 18 + model.n_cam + 1
         n = int(_lib.load().tg_stem_fit_workspace(ctypes.byref(self.geometry))) // 8
         self.workspace = torch.empty(n, dtype=torch.float64, device=device)
-        # its head [np, S]: per-scan-position contributions, one row per output entry
-        self.partials = self.workspace[: self.np * model.n_scan].view(self.np, model.n_scan)
+        # its head [S, np] (row s: scan position s's contributions), exposed
+        # transposed: partials[t, s] is scan position s's share of entry t
+        self.partials = self.workspace[: self.np * model.n_scan].view(model.n_scan, self.np).t()
         self.out = torch.empty(self.np, dtype=torch.float64, device=device)
 
     def run(self, params: torch.Tensor, targets: torch.Tensor) -> torch.Tensor:
