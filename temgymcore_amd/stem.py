@@ -308,6 +308,7 @@ class STEMFitWorkspace:
 
     def terms(self, params: torch.Tensor, targets: torch.Tensor) -> torch.Tensor:
         """Enqueue the per-scan-position terms only; returns the [np, S] view
+        (a transposed view of the workspace's [S, np] rows)
         (row t, column s: scan position s's share of run()'s entry t; the
         last row is the loss per scan position, a residual map)."""
         m = self.model
