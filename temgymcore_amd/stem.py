@@ -307,10 +307,10 @@ class STEMFitWorkspace:
         return self.out
 
     def terms(self, params: torch.Tensor, targets: torch.Tensor) -> torch.Tensor:
-        """Enqueue the per-scan-position terms only; returns the [np, S] view
-        (a transposed view of the workspace's [S, np] rows)
-        (row t, column s: scan position s's share of run()'s entry t; the
-        last row is the loss per scan position, a residual map)."""
+        """Enqueue the per-scan-position terms only; returns an [np, S] view
+        of the workspace's [S, np] rows: row t, column s is scan position s's
+        share of run()'s entry t, and the last row is the loss per scan
+        position, a residual map."""
         m = self.model
         _check_params(m, params)
         shape = (m.n_cam, m.n_scan, m.n_rays, 2)
