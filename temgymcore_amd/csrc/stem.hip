@@ -734,7 +734,18 @@ __global__ __launch_bounds__(kSumThreads) void row_sum_kernel(const double* __re
   __shared__ double red[kSumThreads / 64];
   const double* __restrict__ src = partials + blockIdx.x;  // column blockIdx.x of [rows, cols]
   double acc = 0.0;
-  for (int64_t r = threadIdx.x; r < rows; r += kSumThreads) acc += src[r * cols];
+  int64_t r = threadIdx.x;
+  // four loads in flight per step (a plain loop waited for each), summed in
+  // the same order as one at a time
+  for (; r + 3 * kSumThreads < rows; r += 4 * kSumThreads) {
+    const double a = src[r * cols], b = src[(r + kSumThreads) * cols];
+    const double c = src[(r + 2 * kSumThreads) * cols], d = src[(r + 3 * kSumThreads) * cols];
+    acc += a;
+    acc += b;
+    acc += c;
+    acc += d;
+  }
+  for (; r < rows; r += kSumThreads) acc += src[r * cols];
   acc = wave_sum_f64(acc);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
