@@ -337,15 +337,18 @@ __global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_ste
       acc[k] = 0.0;
       accw[k] = 0.0;
     }
+    // the block's 16 detector values, all loads issued before the first is
+    // used (clamped to the position's first pixel when the ray is off the
+    // detector or past n; `hit` says which rays are real): a load per ray
+    // behind its bounds test waited for each in turn
+    float dv[kBpBlock * kBpBlock];
+    uint32_t hit = 0;
+#pragma unroll
     for (int r = 0; r < kBpBlock; ++r) {
-      const int iy = iy0 + r;
-      if (iy >= n) break;
-      const float ty = th[iy];
 #pragma unroll
       for (int c = 0; c < kBpBlock; ++c) {
-        const int ix = ix0 + c;
-        if (ix >= n) continue;
-        const float tx = th[ix];
+        const int iy = min(iy0 + r, n - 1), ix = min(ix0 + c, n - 1);
+        const float tx = th[ix], ty = th[iy];
         // detector pixel, as in stem_forward_kernel
         const float xd = fmaf(tx, sh_a, sh_bx), yd = fmaf(ty, sh_a, sh_by);
         const double dfy = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[0], (double)yd), __dmul_rn(G.det_m[1], (double)xd)),
@@ -353,8 +356,20 @@ __global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_ste
         const double dfx = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[3], (double)yd), __dmul_rn(G.det_m[4], (double)xd)),
                                      G.det_m[5]);
         const int32_t py = tg::xla_f64_to_i32(rint(dfy)), px = tg::xla_f64_to_i32(rint(dfx));
-        if (py < 0 || py >= G.det_h || px < 0 || px >= G.det_w) continue;
-        const double d = (double)data[(s * G.det_h + py) * G.det_w + px];
+        const bool ok = iy0 + r < n && ix0 + c < n && py >= 0 && py < G.det_h && px >= 0 && px < G.det_w;
+        dv[r * kBpBlock + c] = data[s * G.det_h * G.det_w + (ok ? (int64_t)py * G.det_w + px : 0)];
+        hit |= (uint32_t)ok << (r * kBpBlock + c);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kBpBlock; ++r) {
+      const int iy = min(iy0 + r, n - 1);
+      const float ty = th[iy];
+#pragma unroll
+      for (int c = 0; c < kBpBlock; ++c) {
+        if (!((hit >> (r * kBpBlock + c)) & 1u)) continue;
+        const float tx = th[min(ix0 + c, n - 1)];
+        const double d = (double)dv[r * kBpBlock + c];
         // bilinear weights at the sample-plane position, as in bilinear_tiled()
         double fy, fx;
         sample_pixel(smap, tx, ty, fy, fx);
