@@ -474,10 +474,11 @@ __device__ __forceinline__ void load_tile(const f4* __restrict__ T4, int64_t S, 
 // (FP64 sums of the wavefront partials, waves in order); thread t < np
 // writes entry t of the workspace row s (the np terms of one scan position
 // are contiguous: one short coalesced store per workgroup, where a column
-// layout cost 27 scattered 8-byte stores and ~9 us per pass).  Every term is linear in the
-// moments with coefficients affine in the camera length L_c, so rows 0..17
-// need only the camera sums of the x and y residual totals, plain and
-// L-weighted; row 18 + c takes camera c alone and the last row sums |r|^2.
+// layout cost 27 scattered 8-byte stores and ~9 us per pass).  Every term
+// is linear in the moments with coefficients affine in the camera length
+// L_c, so entries 0..17 need only the camera sums of the x and y residual
+// totals, plain and L-weighted; entry 18 + c takes camera c alone and the
+// last entry sums |r|^2.
 __device__ __forceinline__ void fit_epilogue(const tg_stem_geometry& G, const double* __restrict__ P,
                                              const ScanPos& p, const double (&M)[TG_STEM_MAX_CAM][kMom], int64_t s,
                                              int64_t S, double* __restrict__ partials) {
