@@ -175,7 +175,7 @@ def hbm_read_probe(n_bytes: int = 2 << 30, reps: int = 20):
     lib = _lib.load()
     dev = torch.device("cuda", 0)
     buf = torch.ones(n_bytes // 4, dtype=torch.float32, device=dev)
-    n_blocks = 8 * lib.tg_device_cu_count()
+    n_blocks = 64 * lib.tg_device_cu_count()
     sums = torch.empty(n_blocks, dtype=torch.float32, device=dev)
     stream = _lib.stream_ptr(dev)
 
@@ -186,13 +186,15 @@ def hbm_read_probe(n_bytes: int = 2 << 30, reps: int = 20):
     for _ in range(3):
         run()
     torch.cuda.synchronize()
-    ev = _events(2 * reps)
-    for i in range(reps):
-        ev[2 * i].record()
+    # one event pair around `reps` back-to-back launches (per-launch pairs
+    # add the launch gap to every ~0.3 ms measurement)
+    e0, e1 = _events(2)
+    e0.record()
+    for _ in range(reps):
         run()
-        ev[2 * i + 1].record()
+    e1.record()
     torch.cuda.synchronize()
-    ms = sorted(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(reps))[reps // 2]
+    ms = e0.elapsed_time(e1) / reps
     ok = abs(float(sums.double().sum()) - n_bytes / 4) < 1e-6 * n_bytes
     del buf
     return n_bytes / (ms * 1e-3), ok

@@ -6,7 +6,7 @@
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kUnroll = 8;
+constexpr int kUnroll = 4;  // with ~64 workgroups per CU the fastest of a sweep of read kernels
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(kThreads) void hbm_read_probe_kernel(const f4* __restrict__ src, int64_t n4,
