@@ -649,7 +649,7 @@ def main():
     SKIP_CPU = args.no_cpu_baseline
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the path shards by scan position / detector tile with no exchange; see DESIGN.md)",
+              "(the path would shard by scan position / detector tile with no exchange)",
               file=sys.stderr)
         sys.exit(2)
     res = WORKLOADS[args.workload](args.steps, args.warmup)

@@ -1,4 +1,4 @@
-/* CPU port of the batched propagation path -- TEST / BASELINE INFRASTRUCTURE.
+/* CPU port of the batched propagation path -- TEST AND CPU-BASELINE INFRASTRUCTURE.
  *
  * OpenMP C restatements timed as bench.py's cpu_baseline leg:
  *   plain_run      Lens(z=0.5, f) + Detector(z=1.0) run_to_end + binning, f32

@@ -1,6 +1,6 @@
 """GPU parity of the 4D-STEM kernels against the float64 oracle.
 
-Tolerances (stated per the north-star precision policy):
+Tolerances:
   detector coordinates: |d| <= 1e-6 * max(|ref|, 1e-6 m)   (f32 output)
   sample values:        |d| <= 2e-5 (values in [-pi, pi], f32 bilinear weights)
   fit gradient:         |d| <= 2e-4 * max|g| per block of parameters, vs central
