@@ -93,3 +93,19 @@ def test_fit_workspace_size():
     S, np_, C = 16, 18 + 2 + 1, 2
     assert lib.tg_stem_fit_workspace(ctypes.byref(g)) == 8 * S * (np_ + C * 7)
     assert lib.tg_stem_fit_workspace(None) == 0
+
+
+def test_stem_geometry_cache_copies_and_tracks_fields():
+    """model.geometry() is cached on the model: every call returns a fresh
+    copy equal to a rebuild, and a change of any field it depends on is seen."""
+    model, _ = stem.reference_workload(n_cam=2, scan_shape=(4, 4), n_side=8)
+    g = model.geometry()
+    assert bytes(g) == bytes(model._build_geometry())
+    g.n_side = 7  # mutating a returned struct leaves the cache intact
+    assert model.geometry().n_side == 8
+    for field, value in [("det_pixel", 3e-6), ("camera_lengths", (0.2, 0.9)), ("defocus", 150e-9),
+                         ("sample_pixel", 3e-11), ("descan_offsets", (1e-9, 0.0, 0.0, 2e-6))]:
+        before = bytes(model.geometry())
+        setattr(model, field, value)
+        after = model.geometry()
+        assert bytes(after) == bytes(model._build_geometry()) and bytes(after) != before, field
