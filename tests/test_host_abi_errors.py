@@ -5,7 +5,8 @@ import ctypes
 
 import pytest
 
-from temgymcore_amd import _lib, stem
+# each module declares its entry points' ctypes signatures when imported
+from temgymcore_amd import _lib, gauss_fit, gaussian, source, stem  # noqa: F401
 
 BAD = ctypes.c_void_p(16)  # non-NULL, never dereferenced on these paths
 
@@ -103,7 +104,7 @@ def test_stem_geometry_cache_copies_and_tracks_fields():
     assert bytes(g) == bytes(model._build_geometry())
     g.n_side = 7  # mutating a returned struct leaves the cache intact
     assert model.geometry().n_side == 8
-    for field, value in [("det_pixel", 3e-6), ("camera_lengths", (0.2, 0.9)), ("defocus", 150e-9),
+    for field, value in [("det_pixel", 3e-6), ("camera_lengths", (0.2, 0.5, 0.9)), ("defocus", 150e-9),
                          ("sample_pixel", 3e-11), ("descan_offsets", (1e-9, 0.0, 0.0, 2e-6))]:
         before = bytes(model.geometry())
         setattr(model, field, value)
