@@ -97,7 +97,8 @@ void stem_fit(int scan_h, int scan_w, int n_side, int n_cam, double semi_conv, d
         const double dbx_dy = (A[1] + A[3] * kap) + (A[9] + A[11] * kap) * L;
         const double dby_dx = (A[4] + A[6] * kap) + (A[12] + A[14] * kap) * L;
         const double dby_dy = kap * d + (A[5] + A[7] * kap) + (A[13] + A[15] * kap) * L;
-        const double sx1 = spx / P[16], sy1 = spy / P[16];
+        const double u1 = (double)i - 0.5 * (scan_h - 1), v1 = (double)j - 0.5 * (scan_w - 1);
+        const double sx1 = -sin(phi) * u1 + cos(phi) * v1, sy1 = -(cos(phi) * u1 + sin(phi) * v1);
         const double sx2 = spy * (pi / 180.0), sy2 = -spx * (pi / 180.0);
         g[16] += 2.0 * ((dbx_dx * sx1 + dbx_dy * sy1) * M[0] + (dby_dx * sx1 + dby_dy * sy1) * M[3]);
         g[17] += 2.0 * ((dbx_dx * sx2 + dbx_dy * sy2) * M[0] + (dby_dx * sx2 + dby_dy * sy2) * M[3]);

@@ -518,9 +518,16 @@ __device__ __forceinline__ void fit_epilogue(const tg_stem_geometry& G, const do
       const double ay_x = A[4] + A[6] * kap, by_x = A[12] + A[14] * kap;
       const double ay_y = kap * d + (A[5] + A[7] * kap), by_y = A[13] + A[15] * kap;
       double dsx, dsy;
-      if (t == 16) {  // step: sp is proportional to it
-        dsx = p.spx / P[16];
-        dsy = p.spy / P[16];
+      if (t == 16) {
+        // step: sp = step * (rotated integer pixel offsets), so d sp / d step
+        // is the offsets themselves (finite at step = 0, where sp / step is 0/0)
+        const int64_t i = s / G.scan_w, j = s % G.scan_w;
+        const double u1 = (double)i - 0.5 * (G.scan_h - 1);
+        const double v1 = (double)j - 0.5 * (G.scan_w - 1);
+        double sn, cs;
+        sincos(P[17] * (kPi / 180.0), &sn, &cs);
+        dsx = -sn * u1 + cs * v1;
+        dsy = -(cs * u1 + sn * v1);
       } else {        // rotation in degrees: d sp_x = sp_y dphi, d sp_y = -sp_x dphi
         dsx = p.spy * (kPi / 180.0);
         dsy = -p.spx * (kPi / 180.0);

@@ -244,8 +244,19 @@ def stem_forward(model: STEMModel, params: torch.Tensor | None = None, sample: t
         res["coords"] = torch.empty((C, S, J, 2), dtype=torch.float32, device=device)
     if values and "values" not in res:
         res["values"] = torch.empty((S, J), dtype=torch.float32, device=device)
-    if image4d and "image4d" not in res:
-        res["image4d"] = torch.zeros((S,) + tuple(model.det_shape), dtype=torch.float32, device=device)
+    if image4d:
+        if "image4d" not in res:
+            res["image4d"] = torch.zeros((S,) + tuple(model.det_shape), dtype=torch.float32, device=device)
+        else:
+            # the kernel adds every ray's deposit into the image: a reused
+            # buffer starts from zero on every call (a fill kernel, so the
+            # call stays graph-capturable)
+            img = res["image4d"]
+            if (tuple(img.shape) != (S,) + tuple(model.det_shape) or img.dtype != torch.float32 or not img.is_cuda
+                    or not img.is_contiguous()):
+                raise ValueError(f"out['image4d'] must be a contiguous device float32 tensor of shape "
+                                 f"{(S,) + tuple(model.det_shape)}")
+            img.zero_()
     if (values or image4d) and sample is None:
         raise ValueError("a sample image is needed for values / image4d")
     if sample is not None:

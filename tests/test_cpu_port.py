@@ -45,3 +45,26 @@ def test_stem_fit_port_vs_fd(lib):
     assert out[-1] == pytest.approx(SO.loss(geom, p, targets), rel=1e-9)
     for blk in (slice(0, 16), slice(16, 17), slice(17, 18), slice(18, 20)):
         np.testing.assert_allclose(out[:-1][blk], g[blk], rtol=0, atol=1e-5 * np.abs(g[blk]).max())
+
+
+def test_stem_fit_port_gradient_at_zero_scan_step(lib):
+    # d(scan position)/d(step) is the rotated integer pixel offset, finite at
+    # step 0 (a quotient sp / step would be 0/0 there)
+    geom = dict(scan_shape=(4, 5), n_side=8, semi_conv=0.5e-3, defocus=2e-7, pivot=1e4, offsets=(0.0,) * 4)
+    rng = np.random.default_rng(6)
+    p = np.concatenate([rng.normal(0, 1e-3, 16), [0.0, 12.0], [0.2, 0.9]])
+    q = p.copy()
+    q[16] = 1e-10
+    tx, ty = SO.forward(geom, q)
+    targets = np.ascontiguousarray(np.stack([tx, ty], axis=-1).astype(np.float32))
+    out = np.zeros(21)
+    off = np.zeros(4)
+    lib.stem_fit(4, 5, 8, 2, 0.5e-3, 2e-7, 1e4, off.ctypes.data, p.ctypes.data, targets.ctypes.data, 0, 20,
+                 out.ctypes.data)
+    steps = SO.fd_steps(p)
+    steps[16] = 1e-10
+    g = SO.fd_grad(geom, p, targets, steps=steps)
+    assert np.all(np.isfinite(out))
+    assert abs(g[16]) > 0
+    for blk in (slice(0, 16), slice(16, 17), slice(17, 18), slice(18, 20)):
+        np.testing.assert_allclose(out[:-1][blk], g[blk], rtol=0, atol=1e-5 * np.abs(g[blk]).max())

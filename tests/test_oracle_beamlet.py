@@ -129,3 +129,145 @@ def test_sums_are_linear():
     half = B.make_image(x[:3], y[:3], abcd=abcd, r2=r2, **{k: v[:3] for k, v in args.items()})
     rest = B.make_image(x[3:], y[3:], abcd=abcd, r2=r2, **{k: v[3:] for k, v in args.items()})
     np.testing.assert_allclose(full, half + rest, rtol=1e-12, atol=1e-12 * np.abs(full).max())
+
+
+# ---- input-plane Gaussians (reference gaussian.py:372-427) ------------------------
+#
+# Pins of beamlet_oracle.input_field / input_image, all independent of the
+# matrix form Q = R(theta) diag(1/qx, 1/qy) R(theta)^T the oracle evaluates:
+#  * the reference's own tests: phase 0 at the centre and phase-ramp slope
+#    k*theta (test_gaussians.py:736-786), major axis of a rotated astigmatic
+#    beam at theta (test_gaussians.py:820-883);
+#  * the textbook Gaussian beam written in its principal frame
+#    (rho_1, rho_2) = (d.e1, d.e2), e1 = (cos t, sin t):
+#    |u| = |a| exp(-rho_1^2/wx^2 - rho_2^2/wy^2) and
+#    arg u = arg a + k l + k d.tilt - (k/2) (rho_1^2/Rx + rho_2^2/Ry);
+#  * each planted mistake (tilt term without its factor 2, the transposed
+#    rotation R^T Q R, the pathlength phase with the wrong sign) fails one of
+#    these checks.
+
+def _input_grid(shape=(512, 512), px=1e-5):
+    r1 = B.grid_coords((px, px), shape)
+    return r1, r1.reshape(shape[0], shape[1], 2)
+
+
+def _one_input(x, y, dx, dy, waist, radii=(np.inf, np.inf), theta=0.0, wl=500e-9, amp=1.0 + 0j, path=0.0, r1=None):
+    one = np.ones(1)
+    return B.input_image(x * one, y * one, dx * one, dy * one, path * one, amp * one, np.array([waist], float),
+                         np.array([radii], float), wl * one, theta * one, r1)
+
+
+def _check_phase_ramp():
+    # reference test_gaussians.py:736-786 (512^2 detector, 10 um pixels,
+    # 500 nm, w0 = 100 um, the centre on a pixel centre); the field's phase is
+    # exactly linear for R = inf, so the slopes are asserted to 1e-6 here
+    # instead of the reference's 1-2 %
+    r1, g = _input_grid()
+    iy, ix = 256, 256
+    cx, cy = g[iy, ix]
+    wl, delta = 500e-9, 8
+    k = 2 * np.pi / wl
+    for dx, dy in ((2e-4, 0.0), (0.0, -1e-4), (1.5e-4, -1.0e-4)):
+        img = _one_input(cx, cy, dx, dy, (1e-4, 1e-4), r1=r1).reshape(512, 512)
+        assert abs(np.angle(img[iy, ix])) < 1e-6
+        row = np.unwrap(np.angle(img[iy, :]))
+        col = np.unwrap(np.angle(img[:, ix]))
+        sx = (row[ix + delta] - row[ix - delta]) / (g[iy, ix + delta, 0] - g[iy, ix - delta, 0])
+        sy = (col[iy + delta] - col[iy - delta]) / (g[iy + delta, ix, 1] - g[iy - delta, ix, 1])
+        np.testing.assert_allclose([sx, sy], [k * dx, k * dy], rtol=1e-6, atol=1e-6 * k * 2e-4)
+
+
+def _principal_axis_angle(field, X, Y):
+    inten = np.abs(field) ** 2
+    w = inten.sum()
+    xb, yb = (inten * X).sum() / w, (inten * Y).sum() / w
+    Xc, Yc = X - xb, Y - yb
+    C = np.array([[(inten * Xc * Xc).sum(), (inten * Xc * Yc).sum()],
+                  [(inten * Xc * Yc).sum(), (inten * Yc * Yc).sum()]]) / w
+    vals, vecs = np.linalg.eigh(C)
+    v = vecs[:, np.argmax(vals)]
+    return (np.arctan2(v[1], v[0]) + np.pi / 2) % np.pi - np.pi / 2
+
+
+def _check_rotated_astigmatism():
+    # reference test_gaussians.py:820-883: major axis (w_x = 2 w_y) at theta;
+    # the reference allows 0.1 rad, the intensity moments give it to 1e-6
+    r1, g = _input_grid()
+    X, Y = g[..., 0], g[..., 1]
+    for th in (np.pi / 3, np.pi / 6, 0.0, -np.pi / 6, -np.pi / 3, -np.pi / 2 + 0.1):
+        img = _one_input(0.0, 0.0, 0.0, 0.0, (2e-4, 1e-4), theta=th, r1=r1).reshape(512, 512)
+        phi = _principal_axis_angle(img, X, Y)
+        assert abs(np.arctan2(np.sin(phi - th), np.cos(phi - th))) < 1e-6
+
+
+def _check_principal_frame_closed_form():
+    rng = np.random.default_rng(11)
+    r1 = rng.uniform(-4e-4, 4e-4, (400, 2))
+    wl = 500e-9
+    k = 2 * np.pi / wl
+    cases = [
+        # centre, tilt, waists, radii, theta, amplitude, pathlength
+        ((0.0, 0.0), (0.0, 0.0), (1e-4, 1e-4), (np.inf, np.inf), 0.0, 1.0 + 0j, 0.0),
+        ((3e-5, -5e-5), (2e-4, -1e-4), (1.5e-4, 0.8e-4), (np.inf, np.inf), 0.7, 0.6 - 0.8j, 0.0),
+        ((-2e-5, 4e-5), (-1e-4, 3e-4), (1.2e-4, 2.0e-4), (0.4, -1.3), -1.1, 2.0 + 1.0j, 3.3e-7),
+    ]
+    for (x, y), (tx, ty), (wx, wy), (Rx, Ry), th, a, path in cases:
+        got = _one_input(x, y, tx, ty, (wx, wy), (Rx, Ry), th, wl, a, path, r1)
+        d = r1 - np.array([x, y])
+        rho1 = d[:, 0] * np.cos(th) + d[:, 1] * np.sin(th)
+        rho2 = -d[:, 0] * np.sin(th) + d[:, 1] * np.cos(th)
+        mag = abs(a) * np.exp(-rho1**2 / wx**2 - rho2**2 / wy**2)
+        phase = np.angle(a) + k * path + k * (d @ np.array([tx, ty])) - (k / 2) * (rho1**2 / Rx + rho2**2 / Ry)
+        exp = mag * np.exp(1j * phase)
+        np.testing.assert_allclose(got, exp, rtol=1e-9, atol=1e-12 * abs(a))
+
+
+def _check_sum_is_linear():
+    rng = np.random.default_rng(12)
+    n = 5
+    r1 = rng.uniform(-3e-4, 3e-4, (200, 2))
+    cols = dict(x=rng.uniform(-1e-4, 1e-4, n), y=rng.uniform(-1e-4, 1e-4, n), dx=rng.uniform(-1e-4, 1e-4, n),
+                dy=rng.uniform(-1e-4, 1e-4, n), pathlength=rng.uniform(0, 1e-6, n),
+                amplitude=rng.uniform(0.5, 1, n) * np.exp(1j * rng.uniform(0, 6, n)),
+                waist_xy=rng.uniform(5e-5, 2e-4, (n, 2)), radii=np.full((n, 2), np.inf),
+                wavelength=np.full(n, 500e-9), theta=rng.uniform(-1, 1, n))
+    full = B.input_image(r1=r1, **cols)
+    parts = sum(B.input_image(r1=r1, **{kk: v[i:i + 1] for kk, v in cols.items()}) for i in range(n))
+    np.testing.assert_allclose(full, parts, rtol=1e-12, atol=1e-12 * np.abs(full).max())
+
+
+_INPUT_CHECKS = (_check_phase_ramp, _check_rotated_astigmatism, _check_principal_frame_closed_form,
+                 _check_sum_is_linear)
+
+
+@pytest.mark.parametrize("check", _INPUT_CHECKS, ids=lambda f: f.__name__)
+def test_input_image_pins(check):
+    check()
+
+
+def _planted_tilt_half(mp):
+    orig = B.input_field
+    mp.setattr(B, "input_field", lambda a, p, q1, r1m, t1m, k, r1: orig(a, p, q1, r1m, 0.5 * t1m, k, r1))
+
+
+def _planted_transposed_rotation(mp):
+    orig = B.ray_Q_inv
+    mp.setattr(B, "ray_Q_inv", lambda waist, radii, wl, theta: orig(waist, radii, wl, -np.asarray(theta)))
+
+
+def _planted_pathlength_sign(mp):
+    orig = B.input_field
+    mp.setattr(B, "input_field", lambda a, p, q1, r1m, t1m, k, r1: orig(a, -p, q1, r1m, t1m, k, r1))
+
+
+@pytest.mark.parametrize("plant", [_planted_tilt_half, _planted_transposed_rotation, _planted_pathlength_sign],
+                         ids=lambda f: f.__name__)
+def test_input_image_planted_mistakes_fail(plant, monkeypatch):
+    plant(monkeypatch)
+    caught = 0
+    for check in _INPUT_CHECKS:
+        try:
+            check()
+        except AssertionError:
+            caught += 1
+    assert caught >= 1, "a planted mistake in the input-plane field passed every pin"

@@ -307,3 +307,135 @@ def test_krivanek_planted_mistake_is_caught():
     dW_da, dW_dphi = (gx * ax + gy * ay) / a, gy * ax - gx * ay
     gx_bad = dW_da * ax / a + dW_dphi * ay / a ** 2
     assert not np.allclose(gx_bad, fd, rtol=1e-6, atol=1e-9)
+
+
+# ---- Biprism (reference components.py:526-559) ----------------------------------
+#
+# Pins of ray_oracle.biprism: the reference's tests test_component.py:398-465
+# (the deflection column of the Jacobian for x > 0 and x < 0, the deflection
+# grown over the propagation to the detector, and the lens + biprism system
+# against the product of the analytic 5x5 matrices of tests/transfer_matrices.py
+# written out below), a hand-computed pathlength (incoming slopes), and the
+# identity with a Deflector of +-def_x away from the apex.  Planted mistakes
+# (sign(-x), outgoing slopes in the pathlength) must fail.
+
+def _prop5(z):
+    m = np.eye(5)
+    m[0, 2] = m[1, 3] = z
+    return m
+
+
+def _lens5(f):
+    m = np.eye(5)
+    m[2, 0] = m[3, 1] = -1.0 / f
+    return m
+
+
+def _biprism5(d):
+    m = np.eye(5)
+    m[2, 4] = d
+    return m
+
+
+def _fd_matrix_affine(model, r0, h):
+    """5x5 Jacobian of a model that is affine near r0 (no kink within +-h)."""
+    keys = ("x", "y", "dx", "dy", "one")
+    J = np.zeros((5, 5))
+    for j, k in enumerate(keys):
+        a, b = dict(r0), dict(r0)
+        a[k] = r0[k] + h
+        b[k] = r0[k] - h
+        ra, rb = O.run_to_end(a, model), O.run_to_end(b, model)
+        for i, kk in enumerate(keys):
+            J[i, j] = (ra[kk] - rb[kk]) / (2 * h)
+    return J
+
+
+def _check_biprism_deflection_column():
+    # test_component.py:398-407: d(dx)/d(one) = +def_x just right of the apex;
+    # and -def_x just left of it
+    d = 1e-3
+    for x0, sgn in ((1e-15, 1.0), (-1e-15, -1.0)):
+        J = _fd_matrix_affine([("biprism", 0.0, d)], O.make_ray(x0, 0.0, 0.0, 0.0), 1e-6)
+        assert abs(J[2, 4] - sgn * d) < 1e-12
+        assert abs(J[3, 4]) < 1e-15 and abs(J[0, 4]) < 1e-15
+
+
+def _check_biprism_with_propagation():
+    # test_component.py:410-424: d x_det / d one = def_x (z_det - z_biprism)
+    d, z_det = 1e-3, 0.234
+    for x0, sgn in ((1e-15, 1.0), (-1e-15, -1.0)):
+        J = _fd_matrix_affine([("biprism", 0.0, d), ("detector", z_det)], O.make_ray(x0, 0.0, 0.0, 0.0), 1e-6)
+        assert abs(J[0, 4] - sgn * d * z_det) < 1e-12
+        assert abs(J[2, 4] - sgn * d) < 1e-12
+
+
+def _check_biprism_lens_system():
+    # test_component.py:426-465: source plane at the defocus, lens, biprism
+    # half-way to the detector; ABCD = P(z3) Bp(d) P(z2) L(f) P(z1).  The
+    # system magnifies by -4.5 up to the biprism, so a central ray just left
+    # of the axis at the source crosses the biprism right of its apex; the
+    # Jacobian is taken inside that affine region (source x = -1e-3)
+    M1, F1, defocus, d = -10, 0.0002, 1e-4, 1e-4
+    zl = abs(F1 * (1 / M1 - 1))
+    z2_full = abs(F1 * (1 - M1))
+    zb, zd = zl + z2_full / 2, zl + z2_full
+    model = [("plane", defocus), ("lens", zl, F1), ("biprism", zb, d), ("detector", zd)]
+    r0 = O.make_ray(-1e-3, 0.0, 0.0, 0.0, z=defocus)
+    J = _fd_matrix_affine(model, r0, 1e-5)
+    exp = _prop5(zd - zb) @ _biprism5(d) @ _prop5(zb - zl) @ _lens5(F1) @ _prop5(zl - defocus)
+    np.testing.assert_allclose(J, exp, rtol=1e-8, atol=1e-9)
+    # the central ray itself (-1e-15) is on the same side at the biprism
+    mid = O.run_steps(O.make_ray(-1e-15, 0.0, 0.0, 0.0, z=defocus), model)[4]
+    assert mid["x"] > 0
+
+
+def _check_biprism_pathlength_and_deflector_identity():
+    # pathlength gains dx_in * x + dy_in * y (incoming slopes), by hand:
+    # 0.01 * 0.2 + 0.02 * 0.3 = 0.008
+    r = O.apply(O.make_ray(0.2, 0.3, 0.01, 0.02, pathlength=0.5), ("biprism", 0.0, 1e-3))
+    assert r["pathlength"] == pytest.approx(0.508, abs=1e-15)
+    assert r["dx"] == pytest.approx(0.011, abs=1e-15) and r["dy"] == 0.02
+    # away from the apex the biprism is a Deflector(+-def_x, 0)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, 64)
+    x[np.abs(x) < 1e-3] = 0.5
+    ray = O.make_ray(x, rng.uniform(-1, 1, 64), rng.uniform(-1e-2, 1e-2, 64), rng.uniform(-1e-2, 1e-2, 64),
+                     pathlength=rng.uniform(0, 1, 64))
+    got = O.apply(ray, ("biprism", 0.0, 2e-3))
+    right = O.apply(ray, ("deflector", 0.0, 2e-3, 0.0))
+    left = O.apply(ray, ("deflector", 0.0, -2e-3, 0.0))
+    for k in ("x", "y", "dx", "dy", "pathlength"):
+        np.testing.assert_array_equal(got[k], np.where(x > 0, right[k], left[k]))
+
+
+_BIPRISM_CHECKS = (_check_biprism_deflection_column, _check_biprism_with_propagation, _check_biprism_lens_system,
+                   _check_biprism_pathlength_and_deflector_identity)
+
+
+@pytest.mark.parametrize("check", _BIPRISM_CHECKS, ids=lambda f: f.__name__)
+def test_biprism_pins(check):
+    check()
+
+
+def _planted_biprism_flipped_side(r, def_x):
+    return dict(r, dx=r["dx"] + def_x * r["one"] * np.sign(-r["x"]),
+                pathlength=r["pathlength"] + r["dx"] * r["x"] + r["dy"] * r["y"])
+
+
+def _planted_biprism_outgoing_slopes(r, def_x):
+    dx = r["dx"] + def_x * r["one"] * np.sign(r["x"])
+    return dict(r, dx=dx, pathlength=r["pathlength"] + dx * r["x"] + r["dy"] * r["y"])
+
+
+@pytest.mark.parametrize("plant", [_planted_biprism_flipped_side, _planted_biprism_outgoing_slopes],
+                         ids=lambda f: f.__name__)
+def test_biprism_planted_mistakes_fail(plant, monkeypatch):
+    monkeypatch.setattr(O, "biprism", plant)
+    caught = 0
+    for check in _BIPRISM_CHECKS:
+        try:
+            check()
+        except AssertionError:
+            caught += 1
+    assert caught >= 1, "a planted mistake in the biprism passed every pin"
