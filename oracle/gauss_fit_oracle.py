@@ -60,7 +60,7 @@ def _Q_parts(p):
     return Q, dQ_dqx, dQ_dqy, dQ_dth, dqx_dwx, dqy_dwy, dqx_drx, dqy_dry
 
 
-def grad(params, r1, target, _mistake=None, magnitude=False):
+def grad(params, r1, target, _mistake=None, magnitude=False, rows=None, u=None):
     """dL/d(params) [n, 16] (columns 13-15 are zero).
 
     With u = sum_g u_g and W = (4/N)(|u|^2 - T) conj(u),
@@ -69,17 +69,20 @@ def grad(params, r1, target, _mistake=None, magnitude=False):
     magnitude=True returns sum_pixels |W du_g/dtheta| instead: the size of the
     terms each gradient entry sums, i.e. the scale of its rounding error when
     the terms are summed in finite precision.
+    rows: only these Gaussians' rows are computed (the others stay zero).
+    u: the summed field at r1, if already known (e.g. checked separately on a
+    pixel sample); by default it is field(params, r1).
     `_mistake` plants a wrong term for the finite-difference check to catch."""
     p = np.asarray(params, dtype=np.float64).reshape(-1, NPARAM)
     n = p.shape[0]
     out = np.zeros((n, NPARAM))
     if n == 0:
         return out
-    u = field(p, r1)
+    u = field(p, r1) if u is None else np.asarray(u, dtype=np.complex128)
     N = r1.shape[0]
     W = (4.0 / N) * (np.abs(u) ** 2 - target) * np.conj(u)
     Q, dQqx, dQqy, dQth, dqx_dwx, dqy_dwy, dqx_drx, dqy_dry = _Q_parts(p)
-    for g in range(n):
+    for g in (range(n) if rows is None else rows):
         x, y, tx, ty, pl, ar, ai, _, _, _, _, lam, _ = p[g, :13]
         a = ar + 1j * ai
         k = 2 * np.pi / lam

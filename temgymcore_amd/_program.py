@@ -21,10 +21,14 @@ Data-dependent Python control flow on a ``Sym`` raises ``TypeError``.
 """
 from __future__ import annotations
 
+import collections
 import ctypes
 import dataclasses
 import math
 import numbers
+import threading
+import types
+import weakref
 from typing import Any, Callable
 
 import numpy as np
@@ -310,6 +314,9 @@ class Trace:
         self.instrs: list[_Instr] = []
         self.n_values = 0
         self.slot_tensors: list[torch.Tensor] = []
+        # the object each slot was made from (None for intermediate buffers
+        # of split programs): what a cached plan re-binds on its next call
+        self.slot_source: list = []
         self.slot_value: list[int] = []
         self._slot_by_id: dict[int, int] = {}
         self.deps: list[frozenset] = []   # value id -> slot indices it depends on
@@ -333,6 +340,7 @@ class Trace:
         s = len(self.slot_tensors)
         sym = self._new_value(frozenset((s,)), t.dtype)
         self.slot_tensors.append(t)
+        self.slot_source.append(arr)
         self.slot_value.append(sym.vid)
         self._slot_by_id[key] = s
         self.value_slot[sym.vid] = s
@@ -578,6 +586,7 @@ def _bind_slots(trace: Trace, outs: dict):
     for v, t in outs.items():
         s = len(trace.slot_tensors)
         trace.slot_tensors.append(t)
+        trace.slot_source.append(None)
         trace.slot_value.append(v)
         trace.value_slot[v] = s
 
@@ -645,6 +654,9 @@ def prepare(trace: Trace, comp: Compiled, bin_spec: _lib.BinSpec | None = None):
         prog.out_dtype[k] = _lib.TG_F64 if odt == torch.float64 else _lib.TG_F32
     if bin_spec is not None and comp.bin_regs is not None:
         bin_spec.x_reg, bin_spec.y_reg = comp.bin_regs
+    # what the struct's pointers were bound to (for PlanCache templates)
+    prog.bound_slots = [s for s, _ in comp.slots]
+    prog.bound_outputs = [(v, shape, trace.vdtype[v]) for v, _ in comp.outputs]
     return prog, outs, keep
 
 
@@ -729,15 +741,23 @@ def build_plan(trace: Trace, sym_ray, bin_request=None, extra_rays=()) -> Plan:
         results.update(outs)
         programs.append((prog, spec, keepalive))
     out_vals: list[dict] = [{} for _ in rays]
+    field_src = []  # the primary ray's fields: ("slot", s) | ("vid", v) | ("value", v)
     for (i, name), v in fields.items():
         if (i, name) in direct:
-            out_vals[i][name] = trace.slot_tensors[trace.value_slot[direct[(i, name)]]]
+            s = trace.value_slot[direct[(i, name)]]
+            out_vals[i][name] = trace.slot_tensors[s]
+            src = ("slot", s)
         elif (i, name) in computed:
             out_vals[i][name] = results[computed[(i, name)]]
+            src = ("vid", computed[(i, name)])
         else:
             out_vals[i][name] = v
+            src = ("value", v)
+        if i == 0:
+            field_src.append((name, src))
     plan = Plan(trace, programs, _rebuild(sym_ray, out_vals[0]))
     plan.extra_out = [_rebuild(r, vals) for r, vals in zip(rays[1:], out_vals[1:])]
+    plan.field_src = field_src
     return plan
 
 
@@ -765,3 +785,300 @@ def plan_fn(fn: Callable, ray, bin_request_fn: Callable | None = None) -> Plan:
 def apply_fn(fn: Callable, ray, bin_request_fn: Callable | None = None):
     """Trace, compile and launch `fn` on a vectorised ray; returns the output ray."""
     return plan_fn(fn, ray, bin_request_fn).launch()
+
+
+# --------------------------------------------------------------------------
+# plan cache: repeated drop-in calls skip tracing and compilation
+# --------------------------------------------------------------------------
+
+class Uncacheable(Exception):
+    """A call whose program may depend on something the cache key cannot see."""
+
+
+_ATOM_TYPES = frozenset((type(None), str, bool, torch.device, torch.dtype))
+_DC_INFO: dict = {}        # class -> (field names, frozen) or False
+_FROZEN_KEYS: dict = {}    # id(obj) -> (weakref, key) of frozen, array-free dataclasses
+_FROZEN_KEYS_MAX = 4096
+_MUTABLE = -1              # `seen` entry counting mutable containers walked
+
+
+def _dc_info(t):
+    info = _DC_INFO.get(t)
+    if info is None:
+        info = (tuple(f.name for f in dataclasses.fields(t)), bool(t.__dataclass_params__.frozen)) \
+            if dataclasses.is_dataclass(t) else False
+        _DC_INFO[t] = info
+    return info
+
+
+def _array_key(obj, leaves: list, seen: dict):
+    first = seen.get(id(obj))
+    if first is not None:
+        return ("alias", first)
+    seen[id(obj)] = len(leaves)
+    leaves.append(obj)
+    if isinstance(obj, torch.Tensor):
+        k = (obj.dtype, obj.shape, obj.stride(), obj.device, obj.requires_grad)
+        single = obj.numel() == 1
+    else:
+        k = (obj.dtype.str, obj.shape, obj.strides)
+        single = obj.size == 1
+    if single:
+        v = obj.item()
+        k += (type(v), v.hex() if type(v) is float else v)
+    return k
+
+
+def _structure_key(obj, leaves: list, seen: dict):
+    """Hashable key of everything in `obj` a traced program can depend on.
+
+    Python scalars fold into programs as immediates, so they are keyed by type
+    and exact value.  Arrays become input slots: they are keyed by dtype,
+    shape, strides and device, and collected in `leaves` (in walk order) so a
+    cached plan can bind the next call's arrays.  One-element arrays are also
+    keyed by value (host math may fold them: ``float(p)``, ``.item()``).  An
+    array met again is keyed as an alias of its first occurrence, because the
+    tracer gives each distinct object one slot.  Dataclasses, tuples, lists
+    and dicts are walked, and so are the instance attributes of classes that
+    set ``_plan_key_by_attributes`` (propagators); functions without closures
+    are keyed by identity; anything else is Uncacheable.  The key of a frozen
+    dataclass that holds no arrays and no mutable containers cannot change,
+    so it is remembered per object.
+    """
+    t = type(obj)
+    if t is float:
+        return (float, obj.hex())
+    if t is torch.Tensor or t is np.ndarray:
+        return _array_key(obj, leaves, seen)
+    if t is int:
+        return (int, obj)
+    if t in _ATOM_TYPES:
+        return obj
+    info = _DC_INFO.get(t)
+    if info is None:
+        info = _dc_info(t)
+    if info:
+        names, frozen = info
+        if not frozen:
+            return (t,) + tuple([_structure_key(getattr(obj, n), leaves, seen) for n in names])
+        memo = _FROZEN_KEYS.get(id(obj))
+        if memo is not None and memo[0]() is obj:
+            return memo[1]
+        n_leaves, n_mutable = len(leaves), seen.get(_MUTABLE, 0)
+        key = (t,) + tuple([_structure_key(getattr(obj, n), leaves, seen) for n in names])
+        if len(leaves) == n_leaves and seen.get(_MUTABLE, 0) == n_mutable:
+            if len(_FROZEN_KEYS) >= _FROZEN_KEYS_MAX:
+                _FROZEN_KEYS.clear()
+            try:
+                _FROZEN_KEYS[id(obj)] = (weakref.ref(obj), key)
+            except TypeError:  # no weak references to this class
+                pass
+        return key
+    if isinstance(obj, tuple):
+        return (t,) + tuple([_structure_key(v, leaves, seen) for v in obj])
+    if isinstance(obj, (torch.Tensor, np.ndarray)):
+        return _array_key(obj, leaves, seen)
+    if isinstance(obj, type):
+        return obj
+    if isinstance(obj, numbers.Number):
+        if isinstance(obj, (complex, np.complexfloating)):
+            return (t, float(obj.real).hex(), float(obj.imag).hex())
+        if isinstance(obj, numbers.Integral):
+            return (t, int(obj))
+        return (t, float(obj).hex())
+    if isinstance(obj, list):
+        seen[_MUTABLE] = seen.get(_MUTABLE, 0) + 1
+        return (t,) + tuple([_structure_key(v, leaves, seen) for v in obj])
+    if isinstance(obj, dict):
+        seen[_MUTABLE] = seen.get(_MUTABLE, 0) + 1
+        return (dict,) + tuple((k, _structure_key(obj[k], leaves, seen)) for k in sorted(obj))
+    if isinstance(obj, types.FunctionType) and obj.__closure__ is None:
+        return obj
+    if getattr(t, "_plan_key_by_attributes", False):
+        seen[_MUTABLE] = seen.get(_MUTABLE, 0) + 1
+        return (t, _structure_key(getattr(obj, "__dict__", {}), leaves, seen))
+    raise Uncacheable(t.__name__)
+
+
+class _PlanTemplate:
+    """The compiled programs of one cache key with their pointers unbound."""
+
+    __slots__ = ("device", "programs", "fields", "ray_type", "convert")
+
+    def __init__(self, plan: Plan, leaves: list):
+        trace = plan.trace
+        where = {}
+        for i, leaf in enumerate(leaves):
+            where.setdefault(id(leaf), i)
+        slot_src = []
+        # leaves the tracer converted (host arrays, other dtypes) are converted
+        # again on every call; device tensors it used as they were are bound as-is
+        self.convert = set()
+        for s, src in enumerate(trace.slot_source):
+            if src is None:
+                slot_src.append(("vid", trace.slot_value[s]))
+            elif id(src) in where:
+                slot_src.append(("leaf", where[id(src)]))
+                if trace.slot_tensors[s] is not src:
+                    self.convert.add(where[id(src)])
+            else:
+                raise Uncacheable("an input array is not reachable from the call's arguments")
+        self.device = trace.device
+        self.programs = []
+        for prog, spec, _ in plan.programs:
+            groups: dict = {}
+            for k, (vid, shape, dt) in enumerate(prog.bound_outputs):
+                groups.setdefault((tuple(shape), dt), []).append((k, vid))
+            self.programs.append((_lib.Program.from_buffer_copy(prog),
+                                  None if spec is None else _lib.BinSpec.from_buffer_copy(spec),
+                                  [slot_src[s] for s in prog.bound_slots], list(groups.items())))
+        self.fields = []
+        for name, (kind, x) in plan.field_src:
+            if kind == "slot":
+                src = slot_src[x]
+                if src[0] != "leaf":
+                    raise Uncacheable("an output field is an intermediate buffer")
+                self.fields.append((name, "input", src[1]))
+            elif kind == "vid":
+                self.fields.append((name, "vid", x))
+            elif isinstance(x, (torch.Tensor, np.ndarray)):
+                if id(x) not in where:
+                    raise Uncacheable("an output array is not one of the call's arguments")
+                self.fields.append((name, "leaf", where[id(x)]))
+            elif x is None or isinstance(x, (str, bool, numbers.Number)):
+                self.fields.append((name, "const", x))
+            else:
+                raise Uncacheable(type(x).__name__)
+        self.ray_type = type(plan.out_ray)
+
+    def bind(self, leaves: list, acc: torch.Tensor | None):
+        """Copies of the programs with this call's arrays and fresh outputs
+        bound: ([(Program, BinSpec | None)], output ray, (inputs, outputs)).
+        The outputs of one program that share a shape and dtype are views of
+        one allocation."""
+        dev = self.device
+        inputs: dict[int, torch.Tensor] = {}
+        outs: dict[int, torch.Tensor] = {}
+        convert = self.convert
+
+        def input_tensor(i):
+            t = inputs.get(i)
+            if t is None:
+                t = inputs[i] = to_device_tensor(leaves[i], dev) if i in convert else leaves[i]
+            return t
+
+        bound = []
+        for prog_t, spec_t, binds, groups in self.programs:
+            prog = _lib.Program.from_buffer_copy(prog_t)
+            slots = prog.slot
+            for k, (kind, x) in enumerate(binds):
+                slots[k].ptr = (input_tensor(x) if kind == "leaf" else outs[x]).data_ptr()
+            for (shape, dt), members in groups:
+                if len(members) == 1:
+                    views = (torch.empty(shape, dtype=dt, device=dev),)
+                else:
+                    views = torch.empty((len(members),) + shape, dtype=dt, device=dev).unbind(0)
+                for (k, vid), o in zip(members, views):
+                    outs[vid] = o
+                    prog.out[k] = o.data_ptr()
+            spec = None
+            if spec_t is not None:
+                spec = _lib.BinSpec.from_buffer_copy(spec_t)
+                spec.acc = acc.data_ptr()
+            bound.append((prog, spec))
+        vals = {}
+        for name, kind, x in self.fields:
+            if kind == "vid":
+                vals[name] = outs[x]
+            elif kind == "input":
+                vals[name] = input_tensor(x)
+            elif kind == "leaf":
+                vals[name] = leaves[x]
+            else:
+                vals[name] = x
+        return bound, _rebuild_type(self.ray_type, vals), (inputs, outs)
+
+    def launch(self, leaves: list, acc: torch.Tensor | None, stream: int):
+        """Bind this call's arrays and fresh outputs, launch, return the output ray."""
+        run = _lib.load().tg_ray_program_run
+        bound, out_ray, _ = self.bind(leaves, acc)
+        for prog, spec in bound:
+            st = run(ctypes.byref(prog), None if spec is None else ctypes.byref(spec), stream)
+            if st != 0:
+                _lib.check(st, "tg_ray_program_run")
+        return out_ray
+
+
+_INIT_SPLIT: dict = {}
+
+
+def _rebuild_type(cls, vals: dict):
+    split = _INIT_SPLIT.get(cls)
+    if split is None:
+        fs = dataclasses.fields(cls)
+        split = _INIT_SPLIT[cls] = (tuple(f.name for f in fs if f.init), tuple(f.name for f in fs if not f.init))
+    out = cls(**{n: vals[n] for n in split[0]})
+    for n in split[1]:
+        object.__setattr__(out, n, vals[n])
+    return out
+
+
+class PlanCache:
+    """LRU map from a call's structure key to its compiled programs.
+
+    ``launch(key_objects, build, acc)`` keys `key_objects` (the entry point's
+    name, the model, the propagator, the ray, ...).  On a miss it runs
+    ``build() -> Plan``, launches it and keeps a template; on a hit it binds
+    the call's arrays into copies of the template's programs and launches
+    them, without tracing.  Calls whose structure cannot be keyed, or whose
+    plan reads arrays the key did not see, always take the uncached path.
+    """
+
+    def __init__(self, max_entries: int = 128):
+        self.max_entries = max_entries
+        self._map: "collections.OrderedDict[tuple, _PlanTemplate | None]" = collections.OrderedDict()
+        self._lock = threading.Lock()
+        self.hits = self.misses = self.uncached = 0
+
+    def clear(self):
+        with self._lock:
+            self._map.clear()
+            self.hits = self.misses = self.uncached = 0
+
+    def __len__(self):
+        return len(self._map)
+
+    def launch(self, key_objects: tuple, build: Callable[[], Plan], acc: torch.Tensor | None = None,
+               stream: int | None = None):
+        leaves: list = []
+        try:
+            key = _structure_key(key_objects, leaves, {})
+        except Uncacheable:
+            self.uncached += 1
+            plan = build()
+            return plan.launch(stream)
+        with self._lock:
+            tpl = self._map.get(key, False)
+            if tpl is not False:
+                self._map.move_to_end(key)
+        if tpl is None:  # known to be uncacheable
+            self.uncached += 1
+            return build().launch(stream)
+        if tpl is not False:
+            self.hits += 1
+            st = _lib.stream_ptr(tpl.device) if stream is None else stream
+            return tpl.launch(leaves, acc, st)
+        self.misses += 1
+        plan = build()
+        try:
+            tpl = _PlanTemplate(plan, leaves)
+        except Uncacheable:
+            tpl = None
+        with self._lock:
+            self._map[key] = tpl
+            while len(self._map) > self.max_entries:
+                self._map.popitem(last=False)
+        return plan.launch(stream)
+
+
+plan_cache = PlanCache()

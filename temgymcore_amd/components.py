@@ -31,7 +31,9 @@ def _install_dispatch(cls) -> None:
     @functools.wraps(raw)
     def __call__(self, ray, *args, **kwargs):
         if _program.is_device_ray(ray):
-            return _program.apply_fn(lambda r: raw(self, r, *args, **kwargs), ray)
+            return _program.plan_cache.launch(
+                ("component", raw, self, args, kwargs, ray),
+                lambda: _program.plan_fn(lambda r: raw(self, r, *args, **kwargs), ray))
         return raw(self, ray, *args, **kwargs)
 
     __call__._tg_raw = raw

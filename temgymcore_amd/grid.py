@@ -123,11 +123,16 @@ class Grid:
             # one (scalar) hit, as the reference's numba loop over a 0-d array
             ys = np.atleast_1d(np.asarray(ys, dtype=np.float64))
             xs = np.atleast_1d(np.asarray(xs, dtype=np.float64))
-        spec = make_bin_spec(M, self.shape, acc)
-        trace = _program.Trace(device)
-        sx = trace.lift(xs)
-        sy = trace.lift(ys)
-        _program._materialize(trace, _Pos(sx, sy), (spec, sx, sy))
+        def build():
+            spec = make_bin_spec(M, self.shape, acc)
+            trace = _program.Trace(device)
+            sx = trace.lift(xs)
+            sy = trace.lift(ys)
+            return _program.build_plan(trace, _Pos(sx, sy), (spec, sx, sy))
+
+        # cached by the grid, the position arrays' structure and the
+        # accumulator's: a repeated call re-launches the binning program
+        _program.plan_cache.launch(("into_image", self, isinstance(ray, Ray), xs, ys, acc), build, acc=acc)
         return acc
 
 

@@ -27,10 +27,14 @@ def _propagate_paraxial(ray, distance):
 class BasePropagator:
     """Abstract propagator: implement ``propagate(ray, distance)``."""
 
+    # a propagator's traced program depends on its type and its attributes
+    _plan_key_by_attributes = True
+
     def __call__(self, ray: "Ray", distance) -> "Ray":
         raw = type(self).propagate
         if _program.is_device_ray(ray):
-            return _program.apply_fn(lambda r: raw(r, distance), ray)
+            return _program.plan_cache.launch(("propagate", self, distance, ray),
+                                              lambda: _program.plan_fn(lambda r: raw(r, distance), ray))
         return raw(ray, distance)
 
     @staticmethod
@@ -57,5 +61,6 @@ class FreeSpaceParaxial(BasePropagator):
     @staticmethod
     def propagate(ray, distance):
         if _program.is_device_ray(ray):
-            return _program.apply_fn(lambda r: _propagate_paraxial(r, distance), ray)
+            return _program.plan_cache.launch(
+                ("paraxial", distance, ray), lambda: _program.plan_fn(lambda r: _propagate_paraxial(r, distance), ray))
         return _propagate_paraxial(ray, distance)

@@ -93,11 +93,17 @@ def run_to_end(ray: Ray, components: Sequence, propagator: BasePropagator = Free
     composed=True evaluates an affine model through its composed
     affine-plus-pathlength form (compose.affine_form): one short program
     whatever the model's length, rounding differently from the stepwise
-    evaluation (about 1e-15 relative in float64)."""
-    fn = _model_fn(ray, components, propagator, composed)
+    evaluation (about 1e-15 relative in float64).
+
+    Tensor rays go through the plan cache: a call with the same model,
+    propagator and ray structure (dtypes, shapes, strides, device) as an
+    earlier one re-launches that call's compiled programs on the new arrays
+    without tracing."""
     if _program.is_device_ray(ray):
-        return _program.apply_fn(fn, ray)
-    return fn(ray)
+        return _program.plan_cache.launch(
+            ("run_to_end", tuple(components), propagator, composed, ray),
+            lambda: _program.plan_fn(_model_fn(ray, components, propagator, composed), ray))
+    return _model_fn(ray, components, propagator, composed)(ray)
 
 
 def plan_run_to_image(ray: Ray, components: Sequence, acc=None, grid=None,
@@ -127,10 +133,27 @@ def run_to_image(ray: Ray, components: Sequence, acc=None, grid=None,
     """``run_to_end`` + ``grid.into_image`` fused into one launch.
 
     `grid` defaults to the last component (it must be a Grid, e.g. Detector).
-    Returns (ray_out, acc).
+    Returns (ray_out, acc).  Cached like ``run_to_end`` (the key adds the
+    grid and the accumulator's dtype and shape).
     """
-    plan = plan_run_to_image(ray, components, acc, grid, propagator, composed=composed)
-    return plan.launch(), plan.acc
+    from .grid import Grid, _prepare_acc, make_bin_spec
+
+    grid = components[-1] if grid is None else grid
+    if not isinstance(grid, Grid):
+        raise ValueError("run_to_image needs a Grid (e.g. Detector) to bin into")
+    device = _lib.require_device()
+    acc = _prepare_acc(grid.shape, acc, device)
+    if not _program.is_device_ray(ray):
+        ray = ray.to_vector()
+
+    def build():
+        spec = make_bin_spec(grid.metres_to_pixels_mat, grid.shape, acc)
+        return _program.plan_fn(_model_fn(ray, components, propagator, composed), ray,
+                                bin_request_fn=lambda r: (spec, r.x, r.y))
+
+    out = _program.plan_cache.launch(("run_to_image", tuple(components), grid, propagator, composed, ray, acc),
+                                     build, acc=acc)
+    return out, acc
 
 
 def plan_run_to_end(ray: Ray, components: Sequence, propagator: BasePropagator = FreeSpaceParaxial(), *,

@@ -266,3 +266,39 @@ def test_full_workload_properties_and_sampled_instances(device):
         _field_ok(field[batch.pix_offsets[i]:batch.pix_offsets[i + 1]], u)
         assert l[i] == pytest.approx(L, rel=1e-4)
         _grad_ok(g[batch.gauss_offsets[i]:batch.gauss_offsets[i + 1]], G, S)
+
+
+def test_full_workload_largest_instances(device):
+    """The three instances of the 256-instance workload with the most
+    Gaussians (1,879-1,989 Gaussians on 420^2-508^2 detectors), whose
+    Gaussians are split over several forward slices (fixed-order slice sums,
+    heaviest items first).  At full size:
+      * the field on 4,096 sampled pixels against the oracle;
+      * the loss against the float64 mean over the GPU's own full field;
+      * the gradient rows of 12 sampled Gaussians against the oracle's
+        per-Gaussian derivative, given that same field (the oracle's full
+        field would take ~70 s per instance)."""
+    batch, params, target = gauss_fit.workload()
+    loss, grad = batch.loss_and_grad(params, target)
+    field = batch.field(params).cpu().numpy()
+    p, t = params.cpu().numpy(), target.cpu().numpy()
+    lo, g = loss.cpu().numpy(), grad.cpu().numpy()
+    rng = np.random.default_rng(0)
+    for i in np.argsort(np.asarray(batch.counts), kind="stable")[::-1][:3]:
+        H, W = batch.shapes[i]
+        n = batch.counts[i]
+        print(f"instance {i} shape {(H, W)} n {n}", flush=True)
+        assert n >= 1800
+        r1 = BO.grid_coords((PIX, PIX), (H, W))
+        p0, p1 = batch.pix_offsets[i], batch.pix_offsets[i + 1]
+        g0, g1 = batch.gauss_offsets[i], batch.gauss_offsets[i + 1]
+        u = field[p0:p1].astype(np.complex128)
+        pp = p[g0:g1]
+        sel = rng.choice(H * W, 4096, replace=False)
+        _field_ok(u[sel], GF.field(pp, r1[sel]))
+        tt = t[p0:p1].astype(np.float64)
+        assert lo[i] == pytest.approx(np.mean((np.abs(u) ** 2 - tt) ** 2), rel=1e-5)
+        rows = sorted({0, n - 1} | set(rng.choice(n, 10, replace=False).tolist()))
+        G = GF.grad(pp, r1, tt, rows=rows, u=u)[rows]
+        S = GF.grad(pp, r1, tt, rows=rows, u=u, magnitude=True)[rows]
+        _grad_ok(g[g0:g1][rows], G, S)
