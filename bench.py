@@ -84,9 +84,14 @@ def bench_plain(steps: int, warmup: int):
     from temgymcore_amd import autotune
 
     tuned_grid, tune_ms = autotune.tune_run_to_image(plan)
-    for _ in range(warmup):
+
+    def zero_and_launch():
         plan.acc.zero_()
         plan.launch()
+
+    _settle(zero_and_launch)
+    for _ in range(warmup):
+        zero_and_launch()
     torch.cuda.synchronize()
     # kernel time: events around the launch alone
     ev = _events(2 * steps)
@@ -112,6 +117,7 @@ def bench_plain(steps: int, warmup: int):
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
         step()
+    _settle(graph.replay)
     for _ in range(max(warmup, 1)):
         graph.replay()
     torch.cuda.synchronize()
@@ -128,12 +134,13 @@ def bench_plain(steps: int, warmup: int):
     parity = bool(np.array_equal(plan.out_ray.x.cpu().numpy(), ref["x"]) and
                   np.array_equal(plan.acc.cpu().numpy().astype(np.int64), np.asarray(hist, dtype=np.int64)))
 
-    # CPU baseline: the numpy oracle (single thread) on the full batch, ~10 s
-    reps, c0 = 0, time.perf_counter()
-    while not SKIP_CPU and time.perf_counter() - c0 < 10.0:
-        O.plain_run(x, y, dx, dy)
-        reps += 1
-    cpu_s = (time.perf_counter() - c0) / max(reps, 1)
+    # CPU baseline: the numpy oracle on the full batch, one process per host core
+    cpu = None
+    if not SKIP_CPU:
+        rate, cores = _cpu_parallel("plain", (n,))
+        cpu = {"value": rate, "unit": "rays/s", "cores": cores, "kind": "port",
+               "sample": f"numpy oracle, the full {n}-ray batch repeated for 8 s in each of {cores} "
+                         f"single-threaded processes ({_cpu_info()[1]})"}
     return dict(
         metric="rays/s (plain run_to_image: 1,048,576 f32 rays, Lens + Detector 128x128 binning)",
         value=n / (ms_per_step * 1e-3), unit="rays/s", ms_per_step=ms_per_step, dtype="f32",
@@ -144,9 +151,92 @@ def bench_plain(steps: int, warmup: int):
         roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                   "frac": achieved / HBM_PEAK, "traffic": _profiled_traffic("ray_program_kernel-plain_1M.md"),
                   "traffic_source": "profiles/ray_program_kernel-plain_1M.md (calibrated FETCH_SIZE + WRITE_SIZE)"},
-        cpu_baseline=None if SKIP_CPU else {"value": n / cpu_s, "unit": "rays/s", "cores": 1, "kind": "port",
-                                            "sample": f"numpy oracle, full {n}-ray batch x {reps} reps"},
+        cpu_baseline=cpu,
     )
+
+
+def _cpu_pool_init():
+    # one thread per worker process: the pool spreads the work over the cores
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        os.environ[k] = "1"
+
+
+def _cpu_sample(kind, args):
+    """(one repetition of a workload's oracle sample, work units per repetition)."""
+    import numpy as np
+
+    if kind == "plain":
+        from oracle import ray_oracle as O
+
+        (n,) = args
+        x, y, dx, dy = O.plain_workload(n, seed=0)
+        return (lambda: O.plain_run(x, y, dx, dy)), n
+    if kind == "stem_fwd":
+        from oracle import stem_oracle as SO
+
+        geom, p, n_pos, n_rays = args
+        idx = np.arange(n_pos)
+        return (lambda: (SO.forward(geom, p, scan_idx=idx), SO.sample_plane(geom, p))), n_pos * n_rays
+    if kind == "backproject":
+        from oracle import stem_oracle as SO
+
+        geom, p, D, dT, iT, units = args
+        return (lambda: SO.backproject(geom, p, D, dT, iT)), units
+    if kind == "beamlet":
+        from oracle import beamlet_oracle as BO
+
+        x, y, pix, H, W, L, lam = args
+        nb = len(x)
+        r2 = BO.grid_coords((pix, pix), (H, W))
+        A = np.eye(5)
+        A[0, 2] = A[1, 3] = L
+        d = dict(dx=np.zeros(nb), dy=np.zeros(nb), pathlength=np.zeros(nb), amplitude=np.ones(nb),
+                 waist_xy=np.full((nb, 2), 5e-8), radii=np.full((nb, 2), np.inf), wavelength=np.full(nb, lam),
+                 theta=np.zeros(nb))
+        return (lambda: BO.make_image(x, y, abcd=A, r2=r2, **d)), nb * H * W
+    if kind == "gauss_fit":
+        from oracle import beamlet_oracle as BO
+        from oracle import gauss_fit_oracle as GF
+
+        items = [(p, BO.grid_coords((1e-6, 1e-6), hw), t) for p, hw, t in args]
+
+        def rep():
+            for p, r1, t in items:
+                GF.loss(p, r1, t)
+                GF.grad(p, r1, t)
+
+        return rep, sum(2 * len(p) * len(r1) for p, r1, _ in items)
+    raise ValueError(kind)
+
+
+def _cpu_sample_worker(job):
+    """Repeat one oracle sample from `start_at` (wall clock) for `seconds`;
+    returns (units done, seconds)."""
+    kind, args, start_at, seconds = job
+    rep, units = _cpu_sample(kind, args)
+    rep()  # untimed warm-up
+    while time.time() < start_at:
+        time.sleep(0.001)
+    done, t0 = 0, time.perf_counter()
+    while True:
+        rep()
+        done += units
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            return done, el
+
+
+def _cpu_parallel(kind, args, seconds=8.0):
+    """The oracle sample `kind` on every host core: one single-threaded
+    process per core, all starting together; returns (units/s, cores)."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+
+    cores = len(os.sched_getaffinity(0))
+    start_at = time.time() + 6.0  # after every worker has spawned and warmed up
+    with cf.ProcessPoolExecutor(cores, mp_context=mp.get_context("spawn"), initializer=_cpu_pool_init) as ex:
+        res = list(ex.map(_cpu_sample_worker, [(kind, args, start_at, seconds)] * cores))
+    return sum(u / e for u, e in res), cores
 
 
 def _cpu_info():
@@ -200,11 +290,36 @@ def hbm_read_probe(n_bytes: int = 2 << 30, reps: int = 20):
     return n_bytes / (ms * 1e-3), ok
 
 
-def _time_steps(step_fn, steps, warmup, graph=True):
+SETTLE_MS = 60.0
+
+
+def _settle(step_fn, ms=SETTLE_MS):
+    """Untimed back-to-back steps for about `ms` of GPU time.  After idle the
+    device's power management takes ~30 ms of sustained load to settle, and
+    meanwhile an HBM-bound pass runs up to ~25% slower (measured on the descan
+    fit: 20-launch medians of 181, 214, 192, 180, 174 ... 165 us after 0.5 s
+    idle).  A fitting loop runs in the settled state, which is what the
+    timed steps measure."""
+    import torch
+
+    e0, e1 = _events(2)
+    e0.record()
+    step_fn()
+    e1.record()
+    e1.synchronize()
+    n = int(min(5000, max(1, ms / max(e0.elapsed_time(e1), 1e-3))))
+    for _ in range(n):
+        step_fn()
+    return n
+
+
+def _time_steps(step_fn, steps, warmup, graph=True, settle=True):
     """Time `steps` back-to-back steps on the current stream.
 
     Returns (wall ms/step, GPU ms/step from one HIP event pair around the K
-    steps, GPU ms/step of the same K steps as HIP-graph replays, or None)."""
+    steps, GPU ms/step of the same K steps as HIP-graph replays, or None).
+    With `settle`, ~60 ms of untimed steps (_settle) run before the W warmup
+    steps and before the graph replays."""
     import torch
 
     import gc
@@ -216,6 +331,8 @@ def _time_steps(step_fn, steps, warmup, graph=True):
     # its pauses out of the warmup and the timed region instead
     gc.disable()
     try:
+        if settle:
+            _settle(step_fn)
         for _ in range(max(warmup, 1)):
             step_fn()
         torch.cuda.synchronize()
@@ -240,6 +357,8 @@ def _time_steps(step_fn, steps, warmup, graph=True):
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
         step_fn()
+    if settle:
+        _settle(graph.replay)
     for _ in range(max(warmup, 1)):
         graph.replay()
     torch.cuda.synchronize()
@@ -280,7 +399,9 @@ def bench_stem_fit(steps: int, warmup: int):
     probe_bw, probe_ok = hbm_read_probe()
 
     names = {1: "scan", 2: "camera"}
-    tune_info = {"stem_fit_variant": names[tuned_variant], "candidate_ms": {names[k]: v for k, v in tune_ms.items()}}
+    tune_info = {"stem_fit_variant": names[tuned_variant], "candidate_ms": {names[k]: v for k, v in tune_ms.items()},
+                 "timed_as": "64 back-to-back steps per candidate, best of 3 alternating rounds"}
+    oracle_err = _stem_fit_oracle_error(model, ws, targets) if not SKIP_CPU else None
     if SKIP_CPU:
         res = _result_fit(units, ms_per_step, step_ms, kern_ms, graph_ms, gpu_out, achieved, probe_bw, probe_ok, None,
                           None)
@@ -311,10 +432,37 @@ def bench_stem_fit(steps: int, warmup: int):
                      f"{reps} reps ({cpu_model})"}
     res = _result_fit(units, ms_per_step, step_ms, kern_ms, graph_ms, gpu_out, achieved, probe_bw, probe_ok, cpu, rel)
     res["config"]["autotune"] = tune_info
+    res["config"]["oracle_error"] = oracle_err
     return res
 
 
+def _stem_fit_oracle_error(model, ws, targets, n_pick: int = 4):
+    """Error of the timed result against the float64 oracle, outside the timed
+    region: the GPU's per-scan-position rows (loss + 26 gradient entries, from
+    the timed buffers) summed over `n_pick` seeded scan positions, against the
+    oracle's loss and its central-difference gradient (one Richardson step) on
+    the same f32 targets."""
+    import numpy as np
+
+    from oracle import stem_oracle as SO
+
+    geom = dict(scan_shape=model.scan_shape, n_side=model.n_side, semi_conv=model.semi_conv, defocus=model.defocus,
+                pivot=model.pivot, offsets=tuple(model.descan_offsets))
+    pick = np.sort(np.random.default_rng(0).choice(model.n_scan, n_pick, replace=False))
+    rows = ws.partials.cpu().numpy().T[pick].sum(axis=0)  # [np]: 26 gradient entries, then the loss
+    t = targets.cpu().numpy()[:, pick]
+    ref_loss = SO.loss(geom, model.params(), t, scan_idx=pick)
+    ref_g = SO.fd_grad(geom, model.params(), t, scan_idx=pick)
+    blocks = {"descan_16": slice(0, 16), "scan_step": slice(16, 17), "scan_rotation": slice(17, 18),
+              "camera_lengths_8": slice(18, 18 + model.n_cam)}
+    grad = {k: float(np.max(np.abs(rows[b] - ref_g[b])) / np.max(np.abs(ref_g[b]))) for k, b in blocks.items()}
+    return {"scan_positions": [int(s) for s in pick], "loss_rel": float(abs(rows[-1] - ref_loss) / abs(ref_loss)),
+            "grad_max_abs_err_over_block_max": grad,
+            "oracle": "float64 numpy restatement; gradient by central differences with one Richardson step"}
+
+
 def _result_fit(units, ms_per_step, step_ms, kern_ms, graph_ms, gpu_out, achieved, probe_bw, probe_ok, cpu, rel):
+    step_bw = 8 * units / (ms_per_step * 1e-3)
     return dict(
         metric="rays/s fwd+bwd (16.7M-ray 4D-STEM model x 8 camera lengths: descan/geometry fit loss + 26 grads)",
         value=units / (ms_per_step * 1e-3), unit="ray-evals/s", ms_per_step=ms_per_step, dtype="f32 (f64 reductions)",
@@ -330,7 +478,10 @@ def _result_fit(units, ms_per_step, step_ms, kern_ms, graph_ms, gpu_out, achieve
                   "traffic": _profiled_traffic("stem_fit_kernel-stem_fit_64x64x4096x8.md"),
                   "traffic_source": "profiles/stem_fit_kernel-stem_fit_64x64x4096x8.md (calibrated FETCH_SIZE + WRITE_SIZE)",
                   "measured_read_peak": probe_bw / 1e9 if probe_ok else None,
-                  "frac_of_measured": achieved / probe_bw if probe_ok else None},
+                  "frac_of_measured": achieved / probe_bw if probe_ok else None,
+                  "step_achieved": step_bw / 1e9, "step_frac": step_bw / HBM_PEAK,
+                  "step_note": "the whole timed step (fit pass + final column sums + launch gaps): algorithmic "
+                               "bytes / ms_per_step"},
         cpu_baseline=cpu,
     )
 
@@ -354,14 +505,12 @@ def bench_stem_fwd(steps: int, warmup: int):
     achieved = bytes_per_launch / (kern_ms * 1e-3)
     geom = dict(scan_shape=model.scan_shape, n_side=model.n_side, semi_conv=model.semi_conv,
                 defocus=model.defocus, pivot=model.pivot, offsets=model.descan_offsets)
-    cores, cpu_model = _cpu_info()
-    idx = np.arange(16)
-    reps, c0 = 0, time.perf_counter()
-    while not SKIP_CPU and time.perf_counter() - c0 < 10.0:
-        SO.forward(geom, model.params(), scan_idx=idx)
-        SO.sample_plane(geom, model.params())
-        reps += 1
-    cpu_s = (time.perf_counter() - c0) / max(reps, 1)
+    cpu = None
+    if not SKIP_CPU:
+        rate, cores = _cpu_parallel("stem_fwd", (geom, model.params(), 16, model.n_rays))
+        cpu = {"value": rate, "unit": "rays/s", "cores": cores, "kind": "port",
+               "sample": f"numpy f64 oracle, 16 scan positions x 4096 rays (+ the sample-plane lookup) repeated "
+                         f"for 8 s in each of {cores} single-threaded processes ({_cpu_info()[1]})"}
     return dict(
         metric="rays/s (4D-STEM shadow-image forward: 16.7M rays -> detector coords + bilinear sample values)",
         value=units / (ms_per_step * 1e-3), unit="rays/s", ms_per_step=ms_per_step, dtype="f32",
@@ -372,9 +521,7 @@ def bench_stem_fwd(steps: int, warmup: int):
                   "traffic": _profiled_traffic("stem_forward_kernel-stem_fwd_64x64x4096.md"),
                   "traffic_source": "profiles/stem_forward_kernel-stem_fwd_64x64x4096.md (calibrated FETCH_SIZE + "
                                     "WRITE_SIZE)"},
-        cpu_baseline=None if SKIP_CPU else {
-            "value": 16 * model.n_rays / cpu_s, "unit": "rays/s", "cores": 1, "kind": "port",
-            "sample": f"numpy f64 oracle, 16 scan positions x 4096 rays ({cpu_model})"},
+        cpu_baseline=cpu,
     )
 
 
@@ -408,7 +555,6 @@ def bench_backproject(steps: int, warmup: int):
     achieved = bytes_per_launch / (kern_ms * 1e-3)
     cpu = None
     if not SKIP_CPU:
-        cores, cpu_model = _cpu_info()
         small, _ = stem.reference_workload(n_cam=1, scan_shape=(4, 4))
         geom = dict(scan_shape=small.scan_shape, n_side=small.n_side, semi_conv=small.semi_conv,
                     defocus=small.defocus, pivot=small.pivot, offsets=small.descan_offsets,
@@ -416,13 +562,10 @@ def bench_backproject(steps: int, warmup: int):
         D = data[:small.n_scan].double().cpu().numpy()
         dT = R.metres_to_pixels_matrix((0.0, 0.0), (small.det_pixel,) * 2, small.det_shape)
         iT = R.metres_to_pixels_matrix((0.0, 0.0), (small.sample_pixel,) * 2, small.sample_shape)
-        reps, c0 = 0, time.perf_counter()
-        while time.perf_counter() - c0 < 10.0:
-            SO.backproject(geom, small.params(), D, dT, iT)
-            reps += 1
-        cpu_s = (time.perf_counter() - c0) / reps
-        cpu = {"value": small.n_scan * small.n_rays / cpu_s, "unit": "rays/s", "cores": 1, "kind": "port",
-               "sample": f"numpy f64 oracle, 16 scan positions x 4096 rays ({cpu_model})"}
+        rate, cores = _cpu_parallel("backproject", (geom, small.params(), D, dT, iT, small.n_scan * small.n_rays))
+        cpu = {"value": rate, "unit": "rays/s", "cores": cores, "kind": "port",
+               "sample": f"numpy f64 oracle, 16 scan positions x 4096 rays repeated for 8 s in each of {cores} "
+                         f"single-threaded processes ({_cpu_info()[1]})"}
     return dict(
         metric="rays/s (4D-STEM shadow-image back-projection: 16.7M rays of a 1 GiB dataset onto the sample plane)",
         value=units / (ms_per_step * 1e-3), unit="rays/s", ms_per_step=ms_per_step, dtype="f32 data, f64 sums",
@@ -513,23 +656,13 @@ def bench_beamlet(steps: int, warmup: int):
     evals_per_s = evaluated / (ms_per_step * 1e-3)
     flops_per_eval = 14  # two complex f32 products (6 each) + complex accumulate (2)
     achieved = flops_per_eval * evals_per_s
-    cores, cpu_model = _cpu_info()
     cpu = None
     if not SKIP_CPU:
-        nb, reps, c0 = 16, 0, time.perf_counter()
-        r2 = BO.grid_coords((pix, pix), (H, W))
-        A = np.eye(5)
-        A[0, 2] = A[1, 3] = L
-        d = dict(dx=np.zeros(nb), dy=np.zeros(nb), pathlength=np.zeros(nb), amplitude=np.ones(nb),
-                 waist_xy=np.full((nb, 2), 5e-8), radii=np.full((nb, 2), np.inf), wavelength=np.full(nb, lam),
-                 theta=np.zeros(nb))
-        while time.perf_counter() - c0 < 10.0:
-            BO.make_image(x[:nb], y[:nb], abcd=A, r2=r2, **d)
-            reps += 1
-        cpu_rate = reps * nb * H * W / (time.perf_counter() - c0)
-        cpu = {"value": cpu_rate, "unit": "beamlet-pixel evals/s (dense)", "cores": 1, "kind": "port",
-               "sample": f"numpy complex128 oracle (reference formulas), {nb} beamlets x full 1024^2 detector, "
-                         f"{reps} reps ({cpu_model})"}
+        nb = 16
+        rate, cores = _cpu_parallel("beamlet", (x[:nb], y[:nb], pix, H, W, L, lam))
+        cpu = {"value": rate, "unit": "beamlet-pixel evals/s (dense)", "cores": cores, "kind": "port",
+               "sample": f"numpy complex128 oracle (reference formulas), {nb} beamlets x the full 1024^2 detector "
+                         f"repeated for 8 s in each of {cores} single-threaded processes ({_cpu_info()[1]})"}
     return dict(
         metric="beamlet-pixel evals/s (131,072-beamlet aperture diffraction, 1024^2 detector, 4-sigma cut)",
         value=evals_per_s, unit="beamlet-pixel evals/s", ms_per_step=ms_per_step, dtype="f32 (f64 phase)",
@@ -586,30 +719,24 @@ def bench_gauss_fit(steps: int, warmup: int):
     value = units / (ms_per_step * 1e-3)
     flops_fwd, flops_bwd = 14, 28  # per evaluated pair (recurrence + accumulate; + W product and 3 moments)
     achieved = (flops_fwd + flops_bwd) * evaluated / (kern_ms * 1e-3)
-    cores, cpu_model = _cpu_info()
     cpu = None
     if not SKIP_CPU:
         # the oracle's loss + gradient over the first (up to) 16 Gaussians of
-        # each instance on its full detector, instances in workload order, ~10 s
+        # the first 6 instances, each on its full detector
         p = params.cpu().numpy()
         t = target.cpu().numpy().astype(np.float64)
-        done, c0, i = 0, time.perf_counter(), 0
-        names = []
-        while i < batch.n_inst and (time.perf_counter() - c0 < 10.0 or done == 0):
+        items, names = [], []
+        for i in range(6):
             g0 = int(batch.gauss_offsets[i])
             g1 = min(int(batch.gauss_offsets[i + 1]), g0 + 16)
             h, w = batch.shapes[i]
-            r1 = BO.grid_coords((1e-6, 1e-6), (h, w))
-            tt = t[int(batch.pix_offsets[i]):int(batch.pix_offsets[i + 1])]
-            GF.loss(p[g0:g1], r1, tt)
-            GF.grad(p[g0:g1], r1, tt)
-            done += 2 * (g1 - g0) * h * w
-            names.append(f"{g1 - g0}x{h}^2")
-            i += 1
-        cpu_rate = done / (time.perf_counter() - c0)
-        cpu = {"value": cpu_rate, "unit": "Gaussian-pixel evals/s fwd+bwd (dense)", "cores": 1, "kind": "port",
-               "sample": f"numpy complex128 oracle loss + analytic gradient, first <=16 Gaussians of instances 0..{i - 1} "
-                         f"({', '.join(names[:6])}{'...' if len(names) > 6 else ''}) ({cpu_model})"}
+            items.append((p[g0:g1], (h, w), t[int(batch.pix_offsets[i]):int(batch.pix_offsets[i + 1])]))
+            names.append(f"{g1 - g0}x{h}x{w}")
+        rate, cores = _cpu_parallel("gauss_fit", items)
+        cpu = {"value": rate, "unit": "Gaussian-pixel evals/s fwd+bwd (dense)", "cores": cores, "kind": "port",
+               "sample": f"numpy complex128 oracle loss + analytic gradient, the first <=16 Gaussians of instances "
+                         f"0..5 ({', '.join(names)}), repeated for 8 s in each of {cores} single-threaded processes "
+                         f"({_cpu_info()[1]})"}
     return dict(
         metric="Gaussian-pixel evals/s fwd+bwd (256 batched Gaussian-decomposition fits)",
         value=value, unit="Gaussian-pixel evals/s", ms_per_step=ms_per_step, dtype="f32 (f64 phase, reductions)",
@@ -653,6 +780,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     res = WORKLOADS[args.workload](args.steps, args.warmup)
+    res["config"]["untimed_settle"] = (f"~{SETTLE_MS:.0f} ms of back-to-back untimed steps before the W warmup steps "
+                                       "of every timed measurement: the device's power management settles over "
+                                       "~30 ms of sustained load after idle")
     line = {
         "metric": res["metric"], "value": res["value"], "unit": res["unit"], "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
