@@ -237,6 +237,21 @@ int64_t tg_stem_fit_workspace(const tg_stem_geometry* g);
 int tg_stem_fit(const tg_stem_geometry* g, const double* params, const float* targets, double* workspace,
                 double* out, void* stream);
 
+/* tg_stem_fit without a workspace argument and without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, allocation-free serving and
+ * fitting loops).  The workspace is a device array inside the library, one
+ * per GPU, held until the process ends, of tg_stem_fit_noalloc_max_bytes()
+ * bytes (8 MiB: every 64x64 scan up to TG_STEM_MAX_CAM camera lengths); a
+ * geometry whose tg_stem_fit_workspace(g) exceeds it returns TG_EINVAL before
+ * any launch.  Same kernels and results as tg_stem_fit.  `stream` must belong
+ * to the current device.  Calls on one GPU share its array: calls that overlap
+ * on one GPU (other streams, other threads) race and must be serialised by
+ * the caller. */
+int64_t tg_stem_fit_noalloc_max_bytes(void);
+int tg_stem_fit_noalloc(const tg_stem_geometry* g, const double* params, const float* targets, double* out,
+                        void* stream);
+
 /* ======================================================================== *
  * Gaussian beamlets (reference gaussian.py:225-427)
  *

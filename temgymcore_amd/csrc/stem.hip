@@ -18,6 +18,8 @@
 // follows exactly because the detector coordinates are affine in the slopes.
 // Per-scan-position gradient rows go to a workspace that a second kernel sums
 // in a fixed order (deterministic FP64 result).
+#include <mutex>
+
 #include "tg_common.hpp"
 
 namespace {
@@ -887,6 +889,78 @@ extern "C" int tg_stem_fit(const tg_stem_geometry* g, const double* params, cons
   const int64_t S = (int64_t)g->scan_h * g->scan_w;
   const int np = 18 + g->n_cam + 1;
   hipLaunchKernelGGL(row_sum_kernel, dim3((unsigned)np), dim3(kSumThreads), 0, (hipStream_t)stream, workspace, S,
+                     (int64_t)np, out);
+  TG_LAUNCH_CHECK("row_sum_kernel launch");
+  return TG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// tg_stem_fit_noalloc: tg_stem_fit with its workspace in a device array of
+// this code object instead of a caller buffer, for callers that cannot
+// allocate (HIP graph capture, allocation-free serving loops).  Each GPU holds
+// its own copy of the array until the process ends.  Host code never passes
+// the array's name to a kernel (on the host it is only a stand-in for the
+// symbol): the device address comes from hipGetSymbolAddress, once per
+// device, with that device current.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr int64_t kNoAllocScratchBytes = 8LL << 20;  // 8 MiB per GPU
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchDevices] = {};
+
+// Device address of this device's scratch (the current device, which must be
+// the stream's device).
+int scratch_address(hipStream_t stream, double** out) {
+  int dev = -1;
+  TG_HIP_CHECK(hipGetDevice(&dev));
+  TG_REQUIRE(dev >= 0 && dev < kScratchDevices, "tg_stem_fit_noalloc: device %d out of range", dev);
+  if (stream != nullptr) {
+    int sdev = -1;
+    TG_HIP_CHECK(hipStreamGetDevice(stream, &sdev));
+    TG_REQUIRE(sdev == dev, "tg_stem_fit_noalloc: the stream belongs to device %d, the current device is %d", sdev,
+               dev);
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    TG_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TG_REQUIRE(p != nullptr, "tg_stem_fit_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return TG_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t tg_stem_fit_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
+
+extern "C" int tg_stem_fit_noalloc(const tg_stem_geometry* g, const double* params, const float* targets,
+                                   double* out, void* stream) {
+  tg::clear_error();
+  int st = validate(g, params);
+  if (st != TG_OK) return st;
+  TG_REQUIRE(out != nullptr, "tg_stem_fit_noalloc: NULL buffer");
+  const int64_t need = tg_stem_fit_workspace(g);
+  TG_REQUIRE(need > 0 && need <= kNoAllocScratchBytes,
+             "tg_stem_fit_noalloc: this geometry needs %lld bytes of scratch, more than the %lld available "
+             "(use tg_stem_fit with a caller workspace)",
+             (long long)need, (long long)kNoAllocScratchBytes);
+  double* ws = nullptr;
+  st = scratch_address((hipStream_t)stream, &ws);
+  if (st != TG_OK) return st;
+  // every word of the workspace a launch reads is written earlier in the same
+  // call (per-position rows, per-camera moments): nothing needs clearing
+  st = fit_terms(g, params, targets, ws, (hipStream_t)stream);
+  if (st != TG_OK) return st;
+  const int64_t S = (int64_t)g->scan_h * g->scan_w;
+  const int np = 18 + g->n_cam + 1;
+  hipLaunchKernelGGL(row_sum_kernel, dim3((unsigned)np), dim3(kSumThreads), 0, (hipStream_t)stream, ws, S,
                      (int64_t)np, out);
   TG_LAUNCH_CHECK("row_sum_kernel launch");
   return TG_OK;

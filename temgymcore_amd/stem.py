@@ -51,6 +51,8 @@ _lib.register_signatures({
     "tg_stem_fit": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, _P, _P, _P]),
     "tg_stem_backproject": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, ctypes.c_int32, _P, _P, _P]),
     "tg_stem_fit_terms": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, _P, _P]),
+    "tg_stem_fit_noalloc_max_bytes": (ctypes.c_int64, []),
+    "tg_stem_fit_noalloc": (ctypes.c_int, [ctypes.POINTER(StemGeometry), _P, _P, _P, _P]),
 })
 
 MAX_CAM = 16
