@@ -640,8 +640,10 @@ def bench_beamlet(steps: int, warmup: int):
     from temgymcore_amd import autotune
 
     coef0 = beamlet_coefficients(params, det, MODE_PROPAGATED, abcd, stride, cutoff=4.0)
+    # the tile-sum path (matrix cores for separable beamlets, or the VALU
+    # recurrence), then the beamlet slices under the chosen path
+    tuned_path, path_ms = autotune.tune_beamlet_path(coef0, n, (H, W))
     tuned_splits, tune_ms = autotune.tune_beamlet_sum(coef0, n, (H, W))
-    del coef0
     summer = BeamletSum(n, (H, W), dev)
     holder = {}
 
@@ -651,11 +653,30 @@ def bench_beamlet(steps: int, warmup: int):
         summer.run(coef)
 
     ms_per_step, kern_ms, graph_ms = _time_steps(step, steps, warmup)
-    evaluated = _ellipse_pixel_count(holder["coef"].cpu().numpy().tobytes(), n, H, W)
+    # tg_beamlet_sum alone (path flags, tile sum, slice sum) on the same records
+    _, sum_ms, _ = _time_steps(lambda: summer.run(coef0), steps, warmup, graph=False)
+    rec_bytes = holder["coef"].cpu().numpy().tobytes()
+    evaluated = _ellipse_pixel_count(rec_bytes, n, H, W)
     dense = n * H * W
     evals_per_s = evaluated / (ms_per_step * 1e-3)
-    flops_per_eval = 14  # two complex f32 products (6 each) + complex accumulate (2)
-    achieved = flops_per_eval * evals_per_s
+    matrix = tuned_path == autotune.BEAMLET_PATH_AUTO and _all_separable(rec_bytes)
+    if matrix:
+        # the matrix-core product computes whole 64x64 tiles: 8 flops (one
+        # complex multiply-add) per (beamlet, pixel) of every tile its cutoff box meets
+        tile_pairs = _tile_pair_count(rec_bytes, n, H, W)
+        achieved = 8 * tile_pairs / (sum_ms * 1e-3)
+        roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": 157.3, "unit": "TFLOP/s",
+                "frac": achieved / 157.3e12, "traffic": None, "flops_per_pair": 8, "tile_pairs": tile_pairs,
+                "timed_kernels": "tg_beamlet_sum (path flags + beamlet_sum_sep_kernel + slice sums), HIP events",
+                "peak_note": "MI355X dense FP32 matrix peak (v_mfma_f32_32x32x2f32)"}
+    else:
+        flops_per_eval = 14  # two complex f32 products (6 each) + complex accumulate (2)
+        achieved = flops_per_eval * evaluated / (sum_ms * 1e-3)
+        roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": 157.3, "unit": "TFLOP/s",
+                "frac": achieved / 157.3e12, "traffic": None, "flops_per_eval": flops_per_eval,
+                "valu_busy_percent": _profiled("beamlet_sum_kernel-beamlet_131072_1024sq.md", "valu_busy_percent"),
+                "valu_source": "profiles/beamlet_sum_kernel-beamlet_131072_1024sq.md (VALUBusy PMC): the sum is "
+                               "bound by VALU issue; frac counts only the useful FP32 flops"}
     cpu = None
     if not SKIP_CPU:
         nb = 16
@@ -668,17 +689,48 @@ def bench_beamlet(steps: int, warmup: int):
         value=evals_per_s, unit="beamlet-pixel evals/s", ms_per_step=ms_per_step, dtype="f32 (f64 phase)",
         config={"workload": "beamlet: 131072 beamlets, 10 um aperture, w0 50 nm, 0.3 m, 1024^2 px of 40 nm",
                 "evaluated_pairs": evaluated, "dense_pairs": dense, "dense_equiv_evals_per_s": dense / (ms_per_step * 1e-3),
-                "event_ms_per_step": kern_ms, "graph_replay_ms_per_step": graph_ms,
-                "kernels_per_step": "beamlet_coef_kernel + beamlet_sum_kernel + split_sum_kernel",
-                "autotune": {"beamlet_splits": tuned_splits, "candidate_ms": tune_ms}},
-        roofline={"bound": "valu", "achieved": achieved / 1e12, "peak": 157.3, "unit": "TFLOP/s",
-                  "frac": achieved / 157.3e12, "traffic": None,
-                  "flops_per_eval": flops_per_eval,
-                  "valu_busy_percent": _profiled("beamlet_sum_kernel-beamlet_131072_1024sq.md", "valu_busy_percent"),
-                  "valu_source": "profiles/beamlet_sum_kernel-beamlet_131072_1024sq.md (VALUBusy PMC): the sum is "
-                                 "bound by VALU issue; frac counts only the useful FP32 flops"},
+                "event_ms_per_step": kern_ms, "graph_replay_ms_per_step": graph_ms, "beamlet_sum_event_ms": sum_ms,
+                "tile_sum": "matrix cores (separable beamlets)" if matrix else "VALU recurrence",
+                "kernels_per_step": "beamlet_coef_kernel + sep_flag_kernel + beamlet_sum_sep_kernel / "
+                                    "beamlet_sum_kernel (one exits at once) + split_sum_kernel",
+                "autotune": {"beamlet_path": {1: "auto", 2: "valu"}[tuned_path],
+                             "path_candidate_ms": {{1: "auto", 2: "valu"}[k]: v for k, v in path_ms.items()},
+                             "beamlet_splits": tuned_splits, "candidate_ms": tune_ms}},
+        roofline=roof,
         cpu_baseline=cpu,
     )
+
+
+_COEF_DTYPE = [("phi", "<f8", 12), ("k", "<f4", 2), ("c", "<f4", 2), ("abg", "<f4", 3), ("cut", "<f4"),
+               ("box", "<f4", 2)]
+
+
+def _all_separable(coef_bytes):
+    """Every record separable as the tile-sum path test has it (finite Phi,
+    no px*py term, decaying along both axes)."""
+    import numpy as np
+
+    phi = np.frombuffer(coef_bytes, dtype=np.dtype(_COEF_DTYPE))["phi"]
+    return bool(np.all(np.isfinite(phi)) and np.all(phi[:, 2] == 0) and np.all(phi[:, 3] == 0)
+                and np.all(phi[:, 0] < 0) and np.all(phi[:, 4] < 0))
+
+
+def _tile_pair_count(coef_bytes, n, H, W, tile=64):
+    """(beamlet, pixel) pairs of the 64x64 tiles each beamlet's cutoff box
+    meets (the box test of the tile sums), 4096 per tile."""
+    import numpy as np
+
+    rec = np.frombuffer(coef_bytes, dtype=np.dtype(_COEF_DTYPE))
+    cx, cy = rec["c"][:, 0].astype(np.float64), rec["c"][:, 1].astype(np.float64)
+    bw, bh = rec["box"][:, 0].astype(np.float64), rec["box"][:, 1].astype(np.float64)
+    ntx, nty = -(-W // tile), -(-H // tile)
+    x0 = np.clip(np.ceil((cx - bw - (tile - 1)) / tile), 0, ntx)
+    x1 = np.clip(np.floor((cx + bw) / tile), -1, ntx - 1)
+    y0 = np.clip(np.ceil((cy - bh - (tile - 1)) / tile), 0, nty)
+    y1 = np.clip(np.floor((cy + bh) / tile), -1, nty - 1)
+    nx = np.maximum(x1 - x0 + 1, 0)
+    ny = np.maximum(y1 - y0 + 1, 0)
+    return int((nx * ny).sum()) * tile * tile
 
 
 def bench_gauss_fit(steps: int, warmup: int):

@@ -146,8 +146,21 @@ int32_t tg_device_cu_count(void);
  *   TG_TUNE_FIT_SLICE      key {n_inst, n_gauss, n_pix, 0}: target Gaussians
  *                          per forward slice in tg_gauss_fit_plan, 16..4096
  *   TG_TUNE_STEM_FIT       key {S, J, n_cam, 0}: tg_stem_fit launch variant
- *                          (tg_stem_fit_variant) */
-enum tg_tune_knob { TG_TUNE_BEAMLET_SPLITS = 1, TG_TUNE_RAY_GRID = 2, TG_TUNE_FIT_SLICE = 3, TG_TUNE_STEM_FIT = 4 };
+ *                          (tg_stem_fit_variant)
+ *   TG_TUNE_BEAMLET_PATH   key {n, H, W, 0}: tg_beamlet_sum's tile sum
+ *                          (tg_beamlet_path) */
+enum tg_tune_knob {
+  TG_TUNE_BEAMLET_SPLITS = 1,
+  TG_TUNE_RAY_GRID = 2,
+  TG_TUNE_FIT_SLICE = 3,
+  TG_TUNE_STEM_FIT = 4,
+  TG_TUNE_BEAMLET_PATH = 5
+};
+/* TG_BEAMLET_PATH_AUTO: the matrix-core tile sum when every beamlet of the
+ * launch is separable (no px*py term in Phi, envelope decaying along both
+ * axes), else the VALU recurrence; TG_BEAMLET_PATH_VALU: always the VALU
+ * recurrence. */
+enum tg_beamlet_path { TG_BEAMLET_PATH_AUTO = 1, TG_BEAMLET_PATH_VALU = 2 };
 int tg_tune_set(int32_t knob, const int64_t* key, int32_t value);
 /* 1 and *value set when an entry exists, 0 when not, < 0 on error. */
 int tg_tune_get(int32_t knob, const int64_t* key, int32_t* value);
@@ -299,7 +312,16 @@ int tg_beamlet_coefficients(const double* params, int64_t n, const double* abcd,
 
 /* field[H, W] (complex64 as interleaved f32 pairs) = sum over beamlets of
  * exp(Phi_b(px, py)), overwritten.  `workspace` needs
- * tg_beamlet_sum_workspace(n, H, W) bytes. */
+ * tg_beamlet_sum_workspace(n, H, W) bytes.
+ * Two tile sums (tg_beamlet_path), chosen on the device per launch:
+ *   - every beamlet separable (Phi without a px*py term, envelope decaying
+ *     along both axes, as for untilted round or axis-aligned beams): each
+ *     64x64 tile is a complex matrix product Ey^T Ex over the beamlets whose
+ *     cutoff box overlaps it, Ex/Ey from f64 phases, on the matrix cores
+ *     (f32 MFMA); culling is per tile, so terms below the cut inside an
+ *     overlapping tile are summed too;
+ *   - otherwise the per-lane f32 recurrence over 16-pixel runs, culled per
+ *     run to the cutoff ellipse. */
 int64_t tg_beamlet_sum_workspace(int64_t n, int64_t height, int64_t width);
 int tg_beamlet_sum(const tg_beamlet_coef* coef, int64_t n, int64_t height, int64_t width, float* field,
                    void* workspace, int64_t workspace_bytes, void* stream);

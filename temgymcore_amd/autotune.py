@@ -13,6 +13,8 @@ Knobs (include/temgym_hip.h, ``tg_tune_knob``):
   FIT_SLICE       Gaussians per forward slice of the batched fits
   STEM_FIT        launch variant of the descan fit (per scan position, or
                   per scan position and camera length)
+  BEAMLET_PATH    tile sum of ``tg_beamlet_sum``: the matrix-core product for
+                  separable beamlets (when all are), or the VALU recurrence
 """
 from __future__ import annotations
 
@@ -23,9 +25,11 @@ import torch
 
 from . import _lib
 
-BEAMLET_SPLITS, RAY_GRID, FIT_SLICE, STEM_FIT = 1, 2, 3, 4
-_NAMES = {BEAMLET_SPLITS: "beamlet_splits", RAY_GRID: "ray_grid", FIT_SLICE: "fit_slice", STEM_FIT: "stem_fit"}
+BEAMLET_SPLITS, RAY_GRID, FIT_SLICE, STEM_FIT, BEAMLET_PATH = 1, 2, 3, 4, 5
+_NAMES = {BEAMLET_SPLITS: "beamlet_splits", RAY_GRID: "ray_grid", FIT_SLICE: "fit_slice", STEM_FIT: "stem_fit",
+          BEAMLET_PATH: "beamlet_path"}
 STEM_FIT_SCAN, STEM_FIT_CAMERA = 1, 2
+BEAMLET_PATH_AUTO, BEAMLET_PATH_VALU = 1, 2
 
 
 def _key(key):
@@ -82,6 +86,17 @@ def tune_beamlet_sum(coef: torch.Tensor, n: int, shape, candidates=(8, 16, 32, 6
     summer = BeamletSum(n, shape, coef.device)
     cands = [c for c in candidates if c <= max(n, 1)] or [1]
     return _pick(BEAMLET_SPLITS, (n, shape[0], shape[1]), cands, lambda: summer.run(coef), reps)
+
+
+def tune_beamlet_path(coef: torch.Tensor, n: int, shape, candidates=(BEAMLET_PATH_AUTO, BEAMLET_PATH_VALU),
+                      reps: int = 3):
+    """Tile sum of tg_beamlet_sum(n, H, W) on these coefficient records: the
+    matrix-core product (taken only when every beamlet is separable) or the
+    VALU recurrence."""
+    from .gaussian import BeamletSum
+
+    summer = BeamletSum(n, shape, coef.device)
+    return _pick(BEAMLET_PATH, (n, shape[0], shape[1]), list(candidates), lambda: summer.run(coef), reps)
 
 
 def tune_run_to_image(plan, candidates=None, reps: int = 5):

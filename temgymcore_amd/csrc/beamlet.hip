@@ -51,10 +51,44 @@ __global__ __launch_bounds__(256) void beamlet_coef_kernel(const double* __restr
   out[i] = make_coef(p, T, mode, m, mk(p[5], p[6]), cut);
 }
 
+// ---------------------------------------------------------------------------
+// Path selection: kFlagBlocks workgroups each write whether their share of
+// the beamlets holds one that is not separable (every entry rewritten per
+// launch, no atomics); every tile-sum workgroup reads all of them.
+// ---------------------------------------------------------------------------
+constexpr int kFlagBlocks = 256;
+
+// exp(Phi) = Ex(px) Ey(py) with bounded factors: no px*py term, an envelope
+// that decays along both axes, finite coefficients.
+__device__ __forceinline__ bool separable(const tg_beamlet_coef& c) {
+  bool fin = true;
+#pragma unroll
+  for (int j = 0; j < 12; ++j) fin = fin && isfinite(c.phi[j]);
+  return fin && c.phi[2] == 0.0 && c.phi[3] == 0.0 && c.phi[0] < 0.0 && c.phi[4] < 0.0;
+}
+
+__global__ __launch_bounds__(256) void sep_flag_kernel(const tg_beamlet_coef* __restrict__ coef, int64_t n,
+                                                       int32_t force_valu, int32_t* __restrict__ flags) {
+  int bad = force_valu;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n && !bad; i += (int64_t)gridDim.x * 256)
+    bad = !separable(coef[i]);
+  bad = __syncthreads_or(bad);
+  if (threadIdx.x == 0) flags[blockIdx.x] = bad;
+}
+
+// True (in every thread of the workgroup) when the launch takes the
+// matrix-core path.
+__device__ __forceinline__ bool matrix_path(const int32_t* __restrict__ flags) {
+  const int bad = threadIdx.x < kFlagBlocks ? flags[threadIdx.x] : 0;
+  return !__syncthreads_or(bad);
+}
+
 __global__ __launch_bounds__(kThreads) void beamlet_sum_kernel(const tg_beamlet_coef* __restrict__ coef, int64_t n,
                                                                int64_t H, int64_t W, int32_t ntx, int32_t ntiles,
-                                                               int32_t nsplit, float2* __restrict__ ws) {
+                                                               int32_t nsplit, const int32_t* __restrict__ flags,
+                                                               float2* __restrict__ ws) {
   __shared__ TileLists sh;
+  if (matrix_path(flags)) return;  // beamlet_sum_sep_kernel covers this launch
   const int tile = blockIdx.x % ntiles, split = blockIdx.x / ntiles;
   const int32_t tx0 = (tile % ntx) * kTileW, ty0 = (tile / ntx) * kTileH;
   const int64_t b_begin = n * split / nsplit, b_end = n * (split + 1) / nsplit;
@@ -70,6 +104,140 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_kernel(const tg_beamlet_
 #pragma unroll
     for (int j = 0; j < kRun; ++j)
       if (x0 + j < W) dst[x0 + j] = acc[j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Separable beamlets on the matrix cores.  With b = 0,
+//   exp(Phi) = Ex(px) Ey(py),
+//   Ex = exp(Re a (px - cx)^2 + i Im(a px^2 + d px)),
+//   Ey = exp(Re c (py - cy)^2 + g0 + i Im(c py^2 + e py + f)),
+// cx = -Re d / (2 Re a), cy = -Re e / (2 Re c), g0 = Re f - Re a cx^2 -
+// Re c cy^2 (the log of the peak amplitude), so |Ex| <= 1 and |Ey| <= the
+// peak.  Phases are evaluated in f64 and reduced to revolutions in
+// double-double, as in the recurrence's run starts.  A tile is then the
+// complex matrix product  T[m][n] = sum_k Ey_k(ty0 + m) Ex_k(tx0 + n)  over the
+// beamlets overlapping it: four real f32 MFMA products per k (re re, -im im,
+// re im, im re).  Wave w owns the 32x32 block at rows 32 (w / 2), columns
+// 32 (w % 2) of the 64x64 tile.
+// ---------------------------------------------------------------------------
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kSepK = 32;  // beamlets per LDS chunk (the product's K)
+static_assert(kTileW == 64 && kTileH == 64 && kThreads == 256 && kThreads >= kFlagBlocks,
+              "the matrix-core tile sum assumes 64x64 tiles and four waves");
+
+// One k pair of the complex product: (re, im) += Ey_k(m) Ex_k(n).
+__device__ __forceinline__ void sep_step(const float2 (&ex)[kSepK][kTileW], const float2 (&ey)[kSepK][kTileH], int k,
+                                         int m, int n, floatx16& accr, floatx16& acci) {
+  const float2 a = ey[k][m];
+  const float2 b = ex[k][n];
+  accr = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.x, accr, 0, 0, 0);
+  accr = __builtin_amdgcn_mfma_f32_32x32x2f32(-a.y, b.y, accr, 0, 0, 0);
+  acci = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.y, acci, 0, 0, 0);
+  acci = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.x, acci, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(kThreads) void beamlet_sum_sep_kernel(const tg_beamlet_coef* __restrict__ coef,
+                                                                   int64_t n, int64_t H, int64_t W, int32_t ntx,
+                                                                   int32_t ntiles, int32_t nsplit,
+                                                                   const int32_t* __restrict__ flags,
+                                                                   float2* __restrict__ ws) {
+  __shared__ TileLists sh;
+  __shared__ float2 ex[kSepK][kTileW];  // Ex_k(tx0 + n)
+  __shared__ float2 ey[kSepK][kTileH];  // Ey_k(ty0 + m)
+  __shared__ double xpar[kSepK][4];     // Re a, Im a, Im d, cx
+  __shared__ double ypar[kSepK][6];     // Re c, Im c, Im e, Im f, cy, g0
+  if (!matrix_path(flags)) return;      // beamlet_sum_kernel covers this launch
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int tile = blockIdx.x % ntiles, split = blockIdx.x / ntiles;
+  const int32_t tx0 = (tile % ntx) * kTileW, ty0 = (tile / ntx) * kTileH;
+  const int64_t b_begin = n * split / nsplit, b_end = n * (split + 1) / nsplit;
+  const float ftx0 = (float)tx0, fty0 = (float)ty0, ftx1 = (float)(tx0 + kTileW - 1), fty1 = (float)(ty0 + kTileH - 1);
+  const int mb = (wave >> 1) * 32, nb = (wave & 1) * 32;
+  floatx16 accr, acci;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) accr[j] = acci[j] = 0.0f;
+  for (int64_t b0 = b_begin; b0 < b_end; b0 += kThreads) {
+    // beamlets of this chunk whose cutoff box meets the tile, in order
+    const int64_t b = b0 + t;
+    bool hit = false;
+    if (b < b_end) {
+      const tg_beamlet_coef& c = coef[b];
+      hit = c.cx + c.box_w >= ftx0 && c.cx - c.box_w <= ftx1 && c.cy + c.box_h >= fty0 && c.cy - c.box_h <= fty1;
+    }
+    const uint64_t mask = __ballot(hit);
+    if (lane == 0) sh.wave_cnt[wave] = __popcll(mask);
+    __syncthreads();
+    int off = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      off += (w < wave) ? sh.wave_cnt[w] : 0;
+      total += sh.wave_cnt[w];
+    }
+    if (hit) sh.list[off + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)(b - b0);
+    __syncthreads();
+    for (int q0 = 0; q0 < total; q0 += kSepK) {
+      const int nk = min(kSepK, total - q0);
+      if (t < nk) {
+        const tg_beamlet_coef& c = coef[b0 + sh.list[q0 + t]];
+        const double ar = c.phi[0], cr = c.phi[4];
+        const double cx = -c.phi[6] / (2.0 * ar), cy = -c.phi[8] / (2.0 * cr);
+        xpar[t][0] = ar;
+        xpar[t][1] = c.phi[1];
+        xpar[t][2] = c.phi[7];
+        xpar[t][3] = cx;
+        ypar[t][0] = cr;
+        ypar[t][1] = c.phi[5];
+        ypar[t][2] = c.phi[9];
+        ypar[t][3] = c.phi[11];
+        ypar[t][4] = cy;
+        ypar[t][5] = c.phi[10] - ar * cx * cx - cr * cy * cy;
+      }
+      __syncthreads();
+      // Ex (r < 64) and Ey (r >= 64) of beamlet k; zero past the chunk so an
+      // odd nk adds nothing in the last k pair.  A wavefront's 64 entries
+      // share k and the axis.
+#pragma unroll 4
+      for (int i = 0; i < kSepK * 128 / kThreads; ++i) {
+        const int e = t + i * kThreads, k = e >> 7, r = e & 127;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (k < nk) {
+          if (r < 64) {
+            const double px = (double)(tx0 + r), dx = px - xpar[k][3];
+            v = cexp_f(xpar[k][0] * dx * dx, fma(xpar[k][1], px, xpar[k][2]) * px);
+          } else {
+            const double py = (double)(ty0 + r - 64), dy = py - ypar[k][4];
+            v = cexp_f(fma(ypar[k][0] * dy, dy, ypar[k][5]), fma(fma(ypar[k][1], py, ypar[k][2]), py, ypar[k][3]));
+          }
+        }
+        if (r < 64)
+          ex[k][r] = v;
+        else
+          ey[k][r - 64] = v;
+      }
+      __syncthreads();
+      // A[m][k] = Ey_k(ty0 + mb + m) from lane m + 32 k, B[k][n] = Ex_k(tx0 + nb + n)
+      // from lane n + 32 k (v_mfma_f32_32x32x2f32 operand layout); full
+      // chunks unrolled so the LDS reads run ahead of the products
+      if (nk == kSepK) {
+#pragma unroll
+        for (int kk = 0; kk < kSepK; kk += 2)
+          sep_step(ex, ey, kk + (lane >> 5), mb + (lane & 31), nb + (lane & 31), accr, acci);
+      } else {
+        for (int kk = 0; kk < nk; kk += 2)
+          sep_step(ex, ey, kk + (lane >> 5), mb + (lane & 31), nb + (lane & 31), accr, acci);
+      }
+      __syncthreads();  // the next chunk rewrites ex, ey and the parameters
+    }
+    __syncthreads();  // the next chunk of beamlets rewrites the list
+  }
+  // accumulator j of lane l holds row 8 (j / 4) + 4 (l / 32) + j % 4, column l % 32
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int64_t py = ty0 + mb + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+    const int64_t px = tx0 + nb + (lane & 31);
+    if (py < H && px < W) ws[((int64_t)split * H + py) * W + px] = make_float2(accr[j], acci[j]);
   }
 }
 
@@ -240,9 +408,15 @@ extern "C" int tg_beamlet_coefficients(const double* params, int64_t n, const do
   return TG_OK;
 }
 
+// Workspace of tg_beamlet_sum: [nsplit, H, W] complex64 slice tiles, then the
+// kFlagBlocks path flags (256-byte aligned).
+int64_t slices_bytes(int nsplit, int64_t H, int64_t W) {
+  return ((int64_t)nsplit * H * W * (int64_t)sizeof(float2) + 255) / 256 * 256;
+}
+
 extern "C" int64_t tg_beamlet_sum_workspace(int64_t n, int64_t height, int64_t width) {
   if (n < 0 || height <= 0 || width <= 0) return 0;
-  return (int64_t)splits_for(n, height, width) * height * width * (int64_t)sizeof(float2);
+  return slices_bytes(splits_for(n, height, width), height, width) + kFlagBlocks * (int64_t)sizeof(int32_t);
 }
 
 extern "C" int tg_beamlet_sum(const tg_beamlet_coef* coef, int64_t n, int64_t height, int64_t width, float* field,
@@ -253,14 +427,29 @@ extern "C" int tg_beamlet_sum(const tg_beamlet_coef* coef, int64_t n, int64_t he
   TG_REQUIRE(n == 0 || coef != nullptr, "tg_beamlet_sum: NULL coefficients");
   TG_REQUIRE(n <= (int64_t)1 << 31, "tg_beamlet_sum: too many beamlets");
   const int nsplit = splits_for(n, height, width);
-  const int64_t need = (int64_t)nsplit * height * width * (int64_t)sizeof(float2);
+  const int64_t slices = slices_bytes(nsplit, height, width);
+  const int64_t need = slices + kFlagBlocks * (int64_t)sizeof(int32_t);
   TG_REQUIRE(workspace_bytes >= need, "tg_beamlet_sum: workspace %lld bytes < %lld", (long long)workspace_bytes,
              (long long)need);
   const int64_t ntx = tg::cdiv(width, kTileW), nty = tg::cdiv(height, kTileH);
   const int64_t ntiles = ntx * nty;
   TG_REQUIRE(ntiles * nsplit < (int64_t)1 << 31, "tg_beamlet_sum: detector too large");
+  const int64_t key[4] = {n, height, width, 0};
+  int32_t path = TG_BEAMLET_PATH_AUTO;
+  tg::tune_lookup(TG_TUNE_BEAMLET_PATH, key, path);
+  int32_t* flags = (int32_t*)((char*)workspace + slices);
+  // which tile sum runs is decided on the device (no host round trip, so the
+  // call stays graph-capturable); the other kernel's workgroups exit at once
+  hipLaunchKernelGGL(sep_flag_kernel, dim3(kFlagBlocks), dim3(256), 0, (hipStream_t)stream, coef, n,
+                     (int32_t)(path == TG_BEAMLET_PATH_VALU), flags);
+  TG_LAUNCH_CHECK("sep_flag_kernel launch");
+  hipLaunchKernelGGL(beamlet_sum_sep_kernel, dim3((unsigned)(ntiles * nsplit)), dim3(kThreads), 0,
+                     (hipStream_t)stream, coef, n, height, width, (int32_t)ntx, (int32_t)ntiles, (int32_t)nsplit,
+                     (const int32_t*)flags, (float2*)workspace);
+  TG_LAUNCH_CHECK("beamlet_sum_sep_kernel launch");
   hipLaunchKernelGGL(beamlet_sum_kernel, dim3((unsigned)(ntiles * nsplit)), dim3(kThreads), 0, (hipStream_t)stream,
-                     coef, n, height, width, (int32_t)ntx, (int32_t)ntiles, (int32_t)nsplit, (float2*)workspace);
+                     coef, n, height, width, (int32_t)ntx, (int32_t)ntiles, (int32_t)nsplit, (const int32_t*)flags,
+                     (float2*)workspace);
   TG_LAUNCH_CHECK("beamlet_sum_kernel launch");
   const int64_t npix = height * width;
   hipLaunchKernelGGL(split_sum_kernel, dim3((unsigned)tg::cdiv(npix, 256)), dim3(256), 0, (hipStream_t)stream,

@@ -73,7 +73,7 @@ int32_t tg_device_cu_count(void) { return tg::cu_count(); }
 
 int tg_tune_set(int32_t knob, const int64_t* key, int32_t value) {
   tg::clear_error();
-  TG_REQUIRE(knob >= TG_TUNE_BEAMLET_SPLITS && knob <= TG_TUNE_STEM_FIT, "tg_tune_set: unknown knob %d", knob);
+  TG_REQUIRE(knob >= TG_TUNE_BEAMLET_SPLITS && knob <= TG_TUNE_BEAMLET_PATH, "tg_tune_set: unknown knob %d", knob);
   TG_REQUIRE(key != nullptr, "tg_tune_set: NULL key");
   TG_REQUIRE(value > 0, "tg_tune_set: value must be positive");
   std::lock_guard<std::mutex> lk(g_tune_mu);

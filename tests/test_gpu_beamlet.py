@@ -206,3 +206,57 @@ def test_traced_abcd_matches_host_matrices(device):
     for b in range(6):
         ray = Ray(float(d["x"][b]), float(d["y"][b]), float(d["dx"][b]), float(d["dy"][b]), 0.0, 0.0)
         np.testing.assert_allclose(J[b], np.asarray(ray_matrix(model, ray)), rtol=1e-12, atol=1e-15)
+
+
+def _separable(d):
+    """theta = 0: astigmatic waists and curvatures stay along the axes, so an
+    axis-aligned model leaves Phi without a px*py term."""
+    return dict(d, theta=np.zeros(len(d["x"])))
+
+
+@pytest.mark.parametrize("path", ["auto", "valu"])
+@pytest.mark.parametrize("model_kind", ["free", "lens"])
+def test_separable_beamlets_both_tile_sums_vs_oracle(device, path, model_kind):
+    """Separable beamlets (tilts, offsets, complex amplitudes, astigmatic
+    waists and curvatures along the axes) take the matrix-core tile sum by
+    default; the VALU recurrence, forced through the tuning knob, must give
+    the same field.  150 x 200 pixels: partial edge tiles on both axes."""
+    from temgymcore_amd import autotune
+
+    rng = np.random.default_rng(21)
+    d = _separable(_rays(40, rng))
+    det = Detector(z=1e-3, pixel_size=(3e-8, 3e-8), shape=(150, 200))
+    model = [det] if model_kind == "free" else [Lens(z=4e-4, focal_length=5e-4), det]
+    autotune.clear()
+    if path == "valu":
+        autotune.set_choice(autotune.BEAMLET_PATH, (40, 150, 200), autotune.BEAMLET_PATH_VALU)
+    try:
+        got = make_gaussian_image(_gray(d), model).cpu().numpy()
+    finally:
+        autotune.clear()
+    abcd = ray_matrix(model, Ray(0.0, 0.0, 0.0, 0.0, 0.0, 0.0))
+    ref = BO.make_image(abcd=abcd, r2=BO.grid_coords(det.pixel_size, det.shape), **d).reshape(det.shape)
+    _check(got, ref)
+
+
+def test_one_non_separable_beamlet_selects_the_recurrence(device):
+    """A rotated astigmatic beamlet has a px*py term: the whole launch must
+    take the VALU recurrence (the matrix-core product would drop that term)."""
+    rng = np.random.default_rng(22)
+    d = _separable(_rays(30, rng))
+    d["theta"][7] = 0.6
+    assert d["waist_xy"][7, 0] != d["waist_xy"][7, 1]
+    det = Detector(z=1e-3, pixel_size=(3e-8, 3e-8), shape=(96, 128))
+    got = make_gaussian_image(_gray(d), [det]).cpu().numpy()
+    abcd = ray_matrix([det], Ray(0.0, 0.0, 0.0, 0.0, 0.0, 0.0))
+    ref = BO.make_image(abcd=abcd, r2=BO.grid_coords(det.pixel_size, det.shape), **d).reshape(det.shape)
+    _check(got, ref)
+
+
+def test_separable_input_plane_image_vs_oracle(device):
+    rng = np.random.default_rng(23)
+    d = _separable(_rays(60, rng, lam=2.51e-12, w=2e-6, spread=6e-5, tilt=2e-3, curved=True))
+    grid = Detector(z=0.0, pixel_size=(1e-6, 1e-6), shape=(130, 100))
+    got = evaluate_gaussian_input_image(_gray(d), grid).cpu().numpy()
+    ref = BO.input_image(r1=BO.grid_coords(grid.pixel_size, grid.shape), **d).reshape(grid.shape)
+    _check(got, ref)

@@ -34,10 +34,21 @@ def test_rejects_bad_entries():
 def test_beamlet_workspace_follows_tuned_splits():
     lib = _lib.load()
     n, H, W = 100000, 1024, 1024
+    flags = 256 * 4  # the per-launch tile-sum path flags after the slices
     autotune.set_choice(autotune.BEAMLET_SPLITS, (n, H, W), 16)
-    assert lib.tg_beamlet_sum_workspace(n, H, W) == 16 * H * W * 8
+    assert lib.tg_beamlet_sum_workspace(n, H, W) == 16 * H * W * 8 + flags
     autotune.set_choice(autotune.BEAMLET_SPLITS, (n, H, W), 1000)  # clamped to 128
-    assert lib.tg_beamlet_sum_workspace(n, H, W) == 128 * H * W * 8
+    assert lib.tg_beamlet_sum_workspace(n, H, W) == 128 * H * W * 8 + flags
+    # slices are padded to 256 bytes before the flags
+    assert lib.tg_beamlet_sum_workspace(3, 3, 5) == 256 + flags
+
+
+def test_beamlet_path_knob_is_accepted():
+    key = (1000, 64, 64)
+    autotune.set_choice(autotune.BEAMLET_PATH, key, autotune.BEAMLET_PATH_VALU)
+    assert autotune.get_choice(autotune.BEAMLET_PATH, key) == autotune.BEAMLET_PATH_VALU
+    autotune.clear()
+    assert autotune.get_choice(autotune.BEAMLET_PATH, key) is None
 
 
 def test_fit_plan_follows_tuned_slice():
