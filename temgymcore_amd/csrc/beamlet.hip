@@ -67,11 +67,55 @@ __device__ __forceinline__ bool separable(const tg_beamlet_coef& c) {
   return fin && c.phi[2] == 0.0 && c.phi[3] == 0.0 && c.phi[0] < 0.0 && c.phi[4] < 0.0;
 }
 
+// The factors of a separable beamlet (b = 0):
+//   Ex = exp(ar (px - cx)^2 + i (ai px^2 + di px)),
+//   Ey = exp(cr (py - cy)^2 + g0 + i (ci py^2 + ei py + fi)),
+// with a = ar + i ai etc., cx = -dr / (2 ar), cy = -er / (2 cr) and
+// g0 = fr - ar cx^2 - cr cy^2 (the log of the peak amplitude): |Ex| <= 1 and
+// |Ey| <= the peak.  kx = exp(2a), ky = exp(2c): the ratios of successive
+// one-pixel step ratios along each axis.
+struct SepParams {
+  double ar, ai, di, cx;
+  double cr, ci, ei, fi, cy, g0;
+  float kx_re, kx_im, ky_re, ky_im;
+};
+static_assert(sizeof(SepParams) == 96, "SepParams layout");
+
+__device__ __forceinline__ SepParams sep_params(const tg_beamlet_coef& c) {
+  SepParams q;
+  q.ar = c.phi[0];
+  q.ai = c.phi[1];
+  q.di = c.phi[7];
+  q.cx = -c.phi[6] / (2.0 * q.ar);
+  q.cr = c.phi[4];
+  q.ci = c.phi[5];
+  q.ei = c.phi[9];
+  q.fi = c.phi[11];
+  q.cy = -c.phi[8] / (2.0 * q.cr);
+  q.g0 = c.phi[10] - q.ar * q.cx * q.cx - q.cr * q.cy * q.cy;
+  q.kx_re = c.k_re;
+  q.kx_im = c.k_im;
+  const double ke = exp(2.0 * q.cr);
+  double ks, kc;
+  sincos(2.0 * q.ci, &ks, &kc);
+  q.ky_re = (float)(ke * kc);
+  q.ky_im = (float)(ke * ks);
+  return q;
+}
+
+// flags[blockIdx.x]: some beamlet of this workgroup's share is not separable
+// (or force_valu); par[i]: the factors of beamlet i when it is separable.
 __global__ __launch_bounds__(256) void sep_flag_kernel(const tg_beamlet_coef* __restrict__ coef, int64_t n,
-                                                       int32_t force_valu, int32_t* __restrict__ flags) {
+                                                       int32_t force_valu, int32_t* __restrict__ flags,
+                                                       SepParams* __restrict__ par) {
   int bad = force_valu;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n && !bad; i += (int64_t)gridDim.x * 256)
-    bad = !separable(coef[i]);
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n && !force_valu; i += (int64_t)gridDim.x * 256) {
+    const tg_beamlet_coef& c = coef[i];
+    if (separable(c))
+      par[i] = sep_params(c);
+    else
+      bad = 1;
+  }
   bad = __syncthreads_or(bad);
   if (threadIdx.x == 0) flags[blockIdx.x] = bad;
 }
@@ -108,27 +152,39 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_kernel(const tg_beamlet_
 }
 
 // ---------------------------------------------------------------------------
-// Separable beamlets on the matrix cores.  With b = 0,
-//   exp(Phi) = Ex(px) Ey(py),
-//   Ex = exp(Re a (px - cx)^2 + i Im(a px^2 + d px)),
-//   Ey = exp(Re c (py - cy)^2 + g0 + i Im(c py^2 + e py + f)),
-// cx = -Re d / (2 Re a), cy = -Re e / (2 Re c), g0 = Re f - Re a cx^2 -
-// Re c cy^2 (the log of the peak amplitude), so |Ex| <= 1 and |Ey| <= the
-// peak.  Phases are evaluated in f64 and reduced to revolutions in
-// double-double, as in the recurrence's run starts.  A tile is then the
-// complex matrix product  T[m][n] = sum_k Ey_k(ty0 + m) Ex_k(tx0 + n)  over the
-// beamlets overlapping it: four real f32 MFMA products per k (re re, -im im,
+// Separable beamlets on the matrix cores.  A tile is the complex matrix
+// product  T[m][n] = sum_k Ey_k(ty0 + m) Ex_k(tx0 + n)  over the beamlets whose
+// cutoff box meets it: four real f32 MFMA products per k (re re, -im im,
 // re im, im re).  Wave w owns the 32x32 block at rows 32 (w / 2), columns
-// 32 (w % 2) of the 64x64 tile.
+// 32 (w % 2) of the 64x64 tile.  Ex and Ey of kSepK beamlets are staged in
+// LDS: each thread makes one 16-pixel segment of one factor, starting from
+// its f64 value and one-step ratio (phase reduced to revolutions in
+// double-double) and stepping F <- F R, R <- R k, as the recurrence kernel's
+// runs do.  The next chunk's parameters are loaded while this chunk's
+// factors and products are computed.
 // ---------------------------------------------------------------------------
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-constexpr int kSepK = 32;  // beamlets per LDS chunk (the product's K)
-static_assert(kTileW == 64 && kTileH == 64 && kThreads == 256 && kThreads >= kFlagBlocks,
-              "the matrix-core tile sum assumes 64x64 tiles and four waves");
+constexpr int kSepK = 32;          // beamlets per LDS chunk (the product's K)
+constexpr int kSepSeg = 16;        // pixels per generated segment
+constexpr int kSepRow = kTileW + 1;  // LDS row pitch (float2): a bank offset per beamlet row
+static_assert(kTileW == 64 && kTileH == 64 && kThreads == 256 && kThreads >= kFlagBlocks &&
+                  kSepK * 2 * (kTileW / kSepSeg) == kThreads,
+              "the matrix-core tile sum assumes 64x64 tiles, four waves and one segment per thread");
+
+__device__ __forceinline__ void load_params(const SepParams* __restrict__ par, int64_t i, double2& v0, double2& v1,
+                                            double2& v2, double2& v3, double2& v4, double2& v5) {
+  const double2* src = reinterpret_cast<const double2*>(par + i);
+  v0 = src[0];
+  v1 = src[1];
+  v2 = src[2];
+  v3 = src[3];
+  v4 = src[4];
+  v5 = src[5];
+}
 
 // One k pair of the complex product: (re, im) += Ey_k(m) Ex_k(n).
-__device__ __forceinline__ void sep_step(const float2 (&ex)[kSepK][kTileW], const float2 (&ey)[kSepK][kTileH], int k,
+__device__ __forceinline__ void sep_step(const float2 (&ex)[kSepK][kSepRow], const float2 (&ey)[kSepK][kSepRow], int k,
                                          int m, int n, floatx16& accr, floatx16& acci) {
   const float2 a = ey[k][m];
   const float2 b = ex[k][n];
@@ -139,22 +195,23 @@ __device__ __forceinline__ void sep_step(const float2 (&ex)[kSepK][kTileW], cons
 }
 
 __global__ __launch_bounds__(kThreads) void beamlet_sum_sep_kernel(const tg_beamlet_coef* __restrict__ coef,
-                                                                   int64_t n, int64_t H, int64_t W, int32_t ntx,
-                                                                   int32_t ntiles, int32_t nsplit,
-                                                                   const int32_t* __restrict__ flags,
+                                                                   const SepParams* __restrict__ par, int64_t n,
+                                                                   int64_t H, int64_t W, int32_t ntx, int32_t ntiles,
+                                                                   int32_t nsplit, const int32_t* __restrict__ flags,
                                                                    float2* __restrict__ ws) {
   __shared__ TileLists sh;
-  __shared__ float2 ex[kSepK][kTileW];  // Ex_k(tx0 + n)
-  __shared__ float2 ey[kSepK][kTileH];  // Ey_k(ty0 + m)
-  __shared__ double xpar[kSepK][4];     // Re a, Im a, Im d, cx
-  __shared__ double ypar[kSepK][6];     // Re c, Im c, Im e, Im f, cy, g0
-  if (!matrix_path(flags)) return;      // beamlet_sum_kernel covers this launch
+  __shared__ float2 ex[kSepK][kSepRow];  // Ex_k(tx0 + n)
+  __shared__ float2 ey[kSepK][kSepRow];  // Ey_k(ty0 + m)
+  __shared__ SepParams spar[kSepK];
+  if (!matrix_path(flags)) return;  // beamlet_sum_kernel covers this launch
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int tile = blockIdx.x % ntiles, split = blockIdx.x / ntiles;
   const int32_t tx0 = (tile % ntx) * kTileW, ty0 = (tile / ntx) * kTileH;
   const int64_t b_begin = n * split / nsplit, b_end = n * (split + 1) / nsplit;
   const float ftx0 = (float)tx0, fty0 = (float)ty0, ftx1 = (float)(tx0 + kTileW - 1), fty1 = (float)(ty0 + kTileH - 1);
   const int mb = (wave >> 1) * 32, nb = (wave & 1) * 32;
+  // this thread's segment: beamlet gk of the chunk, axis (waves 0-1: x, 2-3: y), pixels s0 .. s0 + 15
+  const int gk = t & (kSepK - 1), gaxis = t >> 7, s0 = ((t >> 5) & 3) * kSepSeg;
   floatx16 accr, acci;
 #pragma unroll
   for (int j = 0; j < 16; ++j) accr[j] = acci[j] = 0.0f;
@@ -177,57 +234,70 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_sep_kernel(const tg_beam
     }
     if (hit) sh.list[off + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)(b - b0);
     __syncthreads();
+    // threads < kSepK: the next chunk's parameters (six 16-byte words, kept
+    // in named registers: an array here was placed in scratch), in flight
+    double2 n0, n1, n2, n3, n4, n5;
+    if (t < kSepK && t < total) load_params(par, b0 + sh.list[t], n0, n1, n2, n3, n4, n5);
     for (int q0 = 0; q0 < total; q0 += kSepK) {
       const int nk = min(kSepK, total - q0);
       if (t < nk) {
-        const tg_beamlet_coef& c = coef[b0 + sh.list[q0 + t]];
-        const double ar = c.phi[0], cr = c.phi[4];
-        const double cx = -c.phi[6] / (2.0 * ar), cy = -c.phi[8] / (2.0 * cr);
-        xpar[t][0] = ar;
-        xpar[t][1] = c.phi[1];
-        xpar[t][2] = c.phi[7];
-        xpar[t][3] = cx;
-        ypar[t][0] = cr;
-        ypar[t][1] = c.phi[5];
-        ypar[t][2] = c.phi[9];
-        ypar[t][3] = c.phi[11];
-        ypar[t][4] = cy;
-        ypar[t][5] = c.phi[10] - ar * cx * cx - cr * cy * cy;
+        double2* dst = reinterpret_cast<double2*>(&spar[t]);
+        dst[0] = n0;
+        dst[1] = n1;
+        dst[2] = n2;
+        dst[3] = n3;
+        dst[4] = n4;
+        dst[5] = n5;
       }
       __syncthreads();
-      // Ex (r < 64) and Ey (r >= 64) of beamlet k; zero past the chunk so an
-      // odd nk adds nothing in the last k pair.  A wavefront's 64 entries
-      // share k and the axis.
-#pragma unroll 4
-      for (int i = 0; i < kSepK * 128 / kThreads; ++i) {
-        const int e = t + i * kThreads, k = e >> 7, r = e & 127;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (k < nk) {
-          if (r < 64) {
-            const double px = (double)(tx0 + r), dx = px - xpar[k][3];
-            v = cexp_f(xpar[k][0] * dx * dx, fma(xpar[k][1], px, xpar[k][2]) * px);
-          } else {
-            const double py = (double)(ty0 + r - 64), dy = py - ypar[k][4];
-            v = cexp_f(fma(ypar[k][0] * dy, dy, ypar[k][5]), fma(fma(ypar[k][1], py, ypar[k][2]), py, ypar[k][3]));
+      if (t < kSepK && q0 + kSepK + t < total)
+        load_params(par, b0 + sh.list[q0 + kSepK + t], n0, n1, n2, n3, n4, n5);
+      {
+        float2* dst = gaxis == 0 ? &ex[gk][s0] : &ey[gk][s0];
+        if (gk < nk) {
+          // start at the segment's pixel nearest the factor's peak and step
+          // outwards both ways: magnitudes only decay, so an underflowed
+          // start or an overflowing ratio is never stepped through
+          const SepParams& P = spar[gk];
+          const double q2 = gaxis == 0 ? P.ar : P.cr, q1 = gaxis == 0 ? P.ai : P.ci;
+          const double l1 = gaxis == 0 ? P.di : P.ei, c0 = gaxis == 0 ? 0.0 : P.fi, g = gaxis == 0 ? 0.0 : P.g0;
+          const double cen = gaxis == 0 ? P.cx : P.cy;
+          const double p0 = (double)((gaxis == 0 ? tx0 : ty0) + s0);
+          const int js = (int)fmin(fmax(rint(cen - p0), 0.0), (double)(kSepSeg - 1));
+          const double ps = p0 + js, ds = ps - cen;
+          // Phi(p) = q2 (p - cen)^2 + g + i (q1 p^2 + l1 p + c0) along this axis
+          const float2 F = cexp_f(fma(q2 * ds, ds, g), fma(fma(q1, ps, l1), ps, c0));
+          float2 Rf = cexp_f(q2 * fma(2.0, ds, 1.0), fma(q1, fma(2.0, ps, 1.0), l1));    // Phi(p + 1) - Phi(p)
+          float2 Rb = cexp_f(-q2 * fma(2.0, ds, -1.0), -fma(q1, fma(2.0, ps, -1.0), l1));  // Phi(p - 1) - Phi(p)
+          const float2 K = gaxis == 0 ? make_float2(P.kx_re, P.kx_im) : make_float2(P.ky_re, P.ky_im);
+          dst[js] = F;
+          float2 Ff = F, Fb = F;
+#pragma unroll
+          for (int s = 1; s < kSepSeg; ++s) {
+            if (js + s < kSepSeg) {
+              Ff = cmulf(Ff, Rf);
+              Rf = cmulf(Rf, K);
+              dst[js + s] = Ff;
+            }
+            if (js - s >= 0) {
+              Fb = cmulf(Fb, Rb);
+              Rb = cmulf(Rb, K);
+              dst[js - s] = Fb;
+            }
           }
+        } else {
+#pragma unroll
+          for (int j = 0; j < kSepSeg; ++j) dst[j] = make_float2(0.0f, 0.0f);
         }
-        if (r < 64)
-          ex[k][r] = v;
-        else
-          ey[k][r - 64] = v;
       }
       __syncthreads();
       // A[m][k] = Ey_k(ty0 + mb + m) from lane m + 32 k, B[k][n] = Ex_k(tx0 + nb + n)
-      // from lane n + 32 k (v_mfma_f32_32x32x2f32 operand layout); full
-      // chunks unrolled so the LDS reads run ahead of the products
-      if (nk == kSepK) {
+      // from lane n + 32 k (v_mfma_f32_32x32x2f32 operand layout); every
+      // chunk runs all kSepK (zero past nk), unrolled, so the LDS reads run
+      // ahead of the products
 #pragma unroll
-        for (int kk = 0; kk < kSepK; kk += 2)
-          sep_step(ex, ey, kk + (lane >> 5), mb + (lane & 31), nb + (lane & 31), accr, acci);
-      } else {
-        for (int kk = 0; kk < nk; kk += 2)
-          sep_step(ex, ey, kk + (lane >> 5), mb + (lane & 31), nb + (lane & 31), accr, acci);
-      }
+      for (int kk = 0; kk < kSepK; kk += 2)
+        sep_step(ex, ey, kk + (lane >> 5), mb + (lane & 31), nb + (lane & 31), accr, acci);
       __syncthreads();  // the next chunk rewrites ex, ey and the parameters
     }
     __syncthreads();  // the next chunk of beamlets rewrites the list
@@ -408,15 +478,19 @@ extern "C" int tg_beamlet_coefficients(const double* params, int64_t n, const do
   return TG_OK;
 }
 
-// Workspace of tg_beamlet_sum: [nsplit, H, W] complex64 slice tiles, then the
-// kFlagBlocks path flags (256-byte aligned).
+// Workspace of tg_beamlet_sum: [nsplit, H, W] complex64 slice tiles (padded to
+// 256 bytes), the kFlagBlocks path flags, then n SepParams records.
 int64_t slices_bytes(int nsplit, int64_t H, int64_t W) {
   return ((int64_t)nsplit * H * W * (int64_t)sizeof(float2) + 255) / 256 * 256;
 }
 
+int64_t sum_workspace_bytes(int nsplit, int64_t n, int64_t H, int64_t W) {
+  return slices_bytes(nsplit, H, W) + kFlagBlocks * (int64_t)sizeof(int32_t) + n * (int64_t)sizeof(SepParams);
+}
+
 extern "C" int64_t tg_beamlet_sum_workspace(int64_t n, int64_t height, int64_t width) {
   if (n < 0 || height <= 0 || width <= 0) return 0;
-  return slices_bytes(splits_for(n, height, width), height, width) + kFlagBlocks * (int64_t)sizeof(int32_t);
+  return sum_workspace_bytes(splits_for(n, height, width), n, height, width);
 }
 
 extern "C" int tg_beamlet_sum(const tg_beamlet_coef* coef, int64_t n, int64_t height, int64_t width, float* field,
@@ -428,7 +502,7 @@ extern "C" int tg_beamlet_sum(const tg_beamlet_coef* coef, int64_t n, int64_t he
   TG_REQUIRE(n <= (int64_t)1 << 31, "tg_beamlet_sum: too many beamlets");
   const int nsplit = splits_for(n, height, width);
   const int64_t slices = slices_bytes(nsplit, height, width);
-  const int64_t need = slices + kFlagBlocks * (int64_t)sizeof(int32_t);
+  const int64_t need = sum_workspace_bytes(nsplit, n, height, width);
   TG_REQUIRE(workspace_bytes >= need, "tg_beamlet_sum: workspace %lld bytes < %lld", (long long)workspace_bytes,
              (long long)need);
   const int64_t ntx = tg::cdiv(width, kTileW), nty = tg::cdiv(height, kTileH);
@@ -438,14 +512,15 @@ extern "C" int tg_beamlet_sum(const tg_beamlet_coef* coef, int64_t n, int64_t he
   int32_t path = TG_BEAMLET_PATH_AUTO;
   tg::tune_lookup(TG_TUNE_BEAMLET_PATH, key, path);
   int32_t* flags = (int32_t*)((char*)workspace + slices);
+  SepParams* par = (SepParams*)(flags + kFlagBlocks);
   // which tile sum runs is decided on the device (no host round trip, so the
   // call stays graph-capturable); the other kernel's workgroups exit at once
   hipLaunchKernelGGL(sep_flag_kernel, dim3(kFlagBlocks), dim3(256), 0, (hipStream_t)stream, coef, n,
-                     (int32_t)(path == TG_BEAMLET_PATH_VALU), flags);
+                     (int32_t)(path == TG_BEAMLET_PATH_VALU), flags, par);
   TG_LAUNCH_CHECK("sep_flag_kernel launch");
   hipLaunchKernelGGL(beamlet_sum_sep_kernel, dim3((unsigned)(ntiles * nsplit)), dim3(kThreads), 0,
-                     (hipStream_t)stream, coef, n, height, width, (int32_t)ntx, (int32_t)ntiles, (int32_t)nsplit,
-                     (const int32_t*)flags, (float2*)workspace);
+                     (hipStream_t)stream, coef, (const SepParams*)par, n, height, width, (int32_t)ntx,
+                     (int32_t)ntiles, (int32_t)nsplit, (const int32_t*)flags, (float2*)workspace);
   TG_LAUNCH_CHECK("beamlet_sum_sep_kernel launch");
   hipLaunchKernelGGL(beamlet_sum_kernel, dim3((unsigned)(ntiles * nsplit)), dim3(kThreads), 0, (hipStream_t)stream,
                      coef, n, height, width, (int32_t)ntx, (int32_t)ntiles, (int32_t)nsplit, (const int32_t*)flags,

@@ -34,13 +34,15 @@ def test_rejects_bad_entries():
 def test_beamlet_workspace_follows_tuned_splits():
     lib = _lib.load()
     n, H, W = 100000, 1024, 1024
-    flags = 256 * 4  # the per-launch tile-sum path flags after the slices
+    # after the slices: the per-launch tile-sum path flags, then 96 bytes of
+    # separable-factor parameters per beamlet
+    extra = 256 * 4 + 96 * n
     autotune.set_choice(autotune.BEAMLET_SPLITS, (n, H, W), 16)
-    assert lib.tg_beamlet_sum_workspace(n, H, W) == 16 * H * W * 8 + flags
+    assert lib.tg_beamlet_sum_workspace(n, H, W) == 16 * H * W * 8 + extra
     autotune.set_choice(autotune.BEAMLET_SPLITS, (n, H, W), 1000)  # clamped to 128
-    assert lib.tg_beamlet_sum_workspace(n, H, W) == 128 * H * W * 8 + flags
+    assert lib.tg_beamlet_sum_workspace(n, H, W) == 128 * H * W * 8 + extra
     # slices are padded to 256 bytes before the flags
-    assert lib.tg_beamlet_sum_workspace(3, 3, 5) == 256 + flags
+    assert lib.tg_beamlet_sum_workspace(3, 3, 5) == 256 + 256 * 4 + 96 * 3
 
 
 def test_beamlet_path_knob_is_accepted():
