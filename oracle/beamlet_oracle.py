@@ -13,7 +13,13 @@ Pinning (tests/test_oracle_beamlet.py): the reference's closed-form tests
 aligned free-space beamlet); the misaligned terms (shift, tilt) against a
 brute-force Fresnel diffraction integral of the same input Gaussian (up to
 the constant phase the reference drops); e/f offset columns against field
-translation / plane-wave tilt; linearity of the sums.
+translation / plane-wave tilt; linearity of the sums.  input_field /
+input_image: the reference's phase-ramp test (phase 0 at the centre, slope
+k * tilt; test_gaussians.py:736-786) and rotated-astigmatism test (major
+axis at theta; :820-883), the textbook Gaussian beam in its principal frame
+(envelope exp(-rho1^2/wx^2 - rho2^2/wy^2), phase k l + k d.tilt -
+(k/2)(rho1^2/Rx + rho2^2/Ry)), linearity; planted mistakes (tilt without its
+factor 2, the transposed rotation, the pathlength phase sign) each fail.
 """
 from __future__ import annotations
 

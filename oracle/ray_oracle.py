@@ -22,7 +22,14 @@ Pinning (tests/test_oracle_ray.py):
   * pixel-transform known-answer tables (test_component.py:73-174) and the
     rotated scan-step vector (test_component.py:356-378);
   * binning: brute-force per-ray loop == vectorised counts; sum == in-range
-    count; a planted off-by-one in the rounding fails the tables.
+    count; a planted off-by-one in the rounding fails the tables;
+  * biprism: the reference's Jacobian tests (test_component.py:398-465: the
+    deflection column on both sides of the apex, the deflection grown over
+    the propagation, the lens + biprism system against the product of the
+    analytic 5x5 matrices of tests/transfer_matrices.py), a hand-computed
+    pathlength with the incoming slopes, and the identity with a Deflector
+    of +-def_x away from the apex; planted mistakes (sign(-x), outgoing
+    slopes in the pathlength) fail.
 """
 from __future__ import annotations
 

@@ -85,6 +85,55 @@ def test_image4d_deposit(device):
     np.testing.assert_allclose(again["image4d"].cpu().numpy().astype(np.float64), ref, rtol=1e-6, atol=1e-5)
 
 
+def test_image4d_deposit_full_scan_vs_oracle(device):
+    """The 4D deposit of the full 64 x 64 scan (1 GiB of shadow images) on 8
+    sampled scan positions against the float64 oracle's own coordinates and
+    bilinear values (not the kernel's).  A ray whose oracle pixel coordinate
+    lies within 1e-3 pixel of a rounding boundary may land in either
+    neighbour in f32, so its |value| is slack on each candidate pixel; every
+    other pixel matches to the per-ray value tolerance (2e-5, as the values
+    test) times the rays it holds."""
+    print("[stem image4d full scan]", flush=True)
+    model, sample = stem.reference_workload(n_cam=1)
+    img = stem.stem_forward(model, sample=torch.from_numpy(sample).to(device), coords=False,
+                            image4d=True)["image4d"]
+    pick = np.sort(np.random.default_rng(5).choice(model.n_scan, 8, replace=False))
+    got = img[torch.from_numpy(pick).to(device)].cpu().numpy().astype(np.float64)
+    del img
+    geom = _geom(model)
+    x, y = SO.forward(geom, model.params(), scan_idx=pick)
+    x, y = x[0], y[0]
+    xs, ys = SO.sample_plane(geom, model.params())
+    Timg = R.metres_to_pixels_matrix((0.0, 0.0), (model.sample_pixel,) * 2, model.sample_shape)
+    vals = SO.bilinear(sample.astype(np.float64), Timg, xs[pick], ys[pick])
+    Tdet = R.metres_to_pixels_matrix((0.0, 0.0), (model.det_pixel,) * 2, model.det_shape)
+    fy, fx = R.transform(y, x, Tdet)
+    py, px = R.round_to_i32(fy), R.round_to_i32(fx)
+    H, W = model.det_shape
+    near_y = np.abs(fy - np.floor(fy) - 0.5) < 1e-3
+    near_x = np.abs(fx - np.floor(fx) - 0.5) < 1e-3
+    assert (near_y | near_x).mean() < 1e-2
+    for k in range(len(pick)):
+        ref = np.zeros((H, W))
+        cnt = np.zeros((H, W))
+        slack = np.zeros((H, W))
+        ok = (py[k] >= 0) & (py[k] < H) & (px[k] >= 0) & (px[k] < W)
+        firm = ok & ~(near_y[k] | near_x[k])
+        np.add.at(ref, (py[k][firm], px[k][firm]), vals[k][firm])
+        np.add.at(cnt, (py[k][firm], px[k][firm]), 1.0)
+        for j in np.nonzero(near_y[k] | near_x[k])[0]:
+            cy = [int(np.floor(fy[k, j])), int(np.floor(fy[k, j])) + 1] if near_y[k, j] else [int(py[k, j])]
+            cx = [int(np.floor(fx[k, j])), int(np.floor(fx[k, j])) + 1] if near_x[k, j] else [int(px[k, j])]
+            for a in cy:
+                for b in cx:
+                    if 0 <= a < H and 0 <= b < W:
+                        slack[a, b] += abs(vals[k, j]) + 2e-5
+        err = np.abs(got[k] - ref)
+        bound = 2e-5 * cnt + slack + 1e-6
+        assert np.all(err <= bound), f"scan position {pick[k]}: worst excess {np.max(err - bound):.3e}"
+        assert cnt.sum() > 0.5 * model.n_rays
+
+
 def test_generic_component_path_matches_oracle_and_kernel(device):
     """The same model through run_to_end (ray programs) is bit-exact vs the
     component oracle and within tolerance of the fused STEM kernel."""
