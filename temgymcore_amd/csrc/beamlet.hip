@@ -72,34 +72,22 @@ __device__ __forceinline__ bool separable(const tg_beamlet_coef& c) {
 //   Ey = exp(cr (py - cy)^2 + g0 + i (ci py^2 + ei py + fi)),
 // with a = ar + i ai etc., cx = -dr / (2 ar), cy = -er / (2 cr) and
 // g0 = fr - ar cx^2 - cr cy^2 (the log of the peak amplitude): |Ex| <= 1 and
-// |Ey| <= the peak.  kx = exp(2a), ky = exp(2c): the ratios of successive
-// one-pixel step ratios along each axis.
+// |Ey| <= the peak.  One axis is
+//   exp(q2 (p - cen)^2 + g + i (q1 p^2 + l1 p + c0)).
+struct SepAxis {
+  double q2, q1, l1, c0, cen, g;
+};
 struct SepParams {
-  double ar, ai, di, cx;
-  double cr, ci, ei, fi, cy, g0;
-  float kx_re, kx_im, ky_re, ky_im;
+  SepAxis x, y;
 };
 static_assert(sizeof(SepParams) == 96, "SepParams layout");
 
 __device__ __forceinline__ SepParams sep_params(const tg_beamlet_coef& c) {
   SepParams q;
-  q.ar = c.phi[0];
-  q.ai = c.phi[1];
-  q.di = c.phi[7];
-  q.cx = -c.phi[6] / (2.0 * q.ar);
-  q.cr = c.phi[4];
-  q.ci = c.phi[5];
-  q.ei = c.phi[9];
-  q.fi = c.phi[11];
-  q.cy = -c.phi[8] / (2.0 * q.cr);
-  q.g0 = c.phi[10] - q.ar * q.cx * q.cx - q.cr * q.cy * q.cy;
-  q.kx_re = c.k_re;
-  q.kx_im = c.k_im;
-  const double ke = exp(2.0 * q.cr);
-  double ks, kc;
-  sincos(2.0 * q.ci, &ks, &kc);
-  q.ky_re = (float)(ke * kc);
-  q.ky_im = (float)(ke * ks);
+  const double ar = c.phi[0], cr = c.phi[4];
+  const double cx = -c.phi[6] / (2.0 * ar), cy = -c.phi[8] / (2.0 * cr);
+  q.x = {ar, c.phi[1], c.phi[7], 0.0, cx, 0.0};
+  q.y = {cr, c.phi[5], c.phi[9], c.phi[11], cy, c.phi[10] - ar * cx * cx - cr * cy * cy};
   return q;
 }
 
@@ -156,38 +144,90 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_kernel(const tg_beamlet_
 // product  T[m][n] = sum_k Ey_k(ty0 + m) Ex_k(tx0 + n)  over the beamlets whose
 // cutoff box meets it: four real f32 MFMA products per k (re re, -im im,
 // re im, im re).  Wave w owns the 32x32 block at rows 32 (w / 2), columns
-// 32 (w % 2) of the 64x64 tile.  Ex and Ey of kSepK beamlets are staged in
-// LDS: each thread makes one 16-pixel segment of one factor, starting from
-// its f64 value and one-step ratio (phase reduced to revolutions in
-// double-double) and stepping F <- F R, R <- R k, as the recurrence kernel's
-// runs do.  The next chunk's parameters are loaded while this chunk's
-// factors and products are computed.
+// 32 (w % 2) of the 64x64 tile.  The factors of kSepK beamlets at a time are
+// staged in LDS, double-buffered: a wave evaluates chunk c + 1 into one
+// buffer while issuing chunk c's products from the other, one barrier per
+// chunk.  (On gfx950 the vector instructions and the f32 MFMAs of a SIMD do
+// not execute concurrently -- measured: their times add, also across waves
+// -- so the evaluation's instruction count is what it costs.)  Each thread
+// evaluates one 16-pixel segment of one factor directly, without branches:
+//   exp2(q2' (p - cen)^2 + g') (cos, sin)(2 pi theta(p)),
+//   theta(p0 + j) = theta0 + j A + j^2 B   (revolutions),
+// theta0 being the segment start's phase reduced in double-double.  theta0,
+// A and B are held modulo one revolution as 32-bit fixed-point fractions and
+// theta is stepped by its exact second difference with wrapping integer
+// adds, so a value's phase is good to ~1e-7 revolutions whatever the size
+// of the linear and quadratic phase terms (an f32 sum lost 1e-5 revolutions
+// on strongly curved input-plane beamlets).  Parameters are loaded one chunk
+// ahead of their use.
 // ---------------------------------------------------------------------------
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int kSepK = 32;          // beamlets per LDS chunk (the product's K)
 constexpr int kSepSeg = 16;        // pixels per generated segment
 constexpr int kSepRow = kTileW + 1;  // LDS row pitch (float2): a bank offset per beamlet row
+constexpr int kSepBlock = 1024;    // beamlets per list build
+constexpr double kLog2e = 1.4426950408889634;
 static_assert(kTileW == 64 && kTileH == 64 && kThreads == 256 && kThreads >= kFlagBlocks &&
-                  kSepK * 2 * (kTileW / kSepSeg) == kThreads,
-              "the matrix-core tile sum assumes 64x64 tiles, four waves and one segment per thread");
+                  kSepK * 2 * (kTileW / kSepSeg) == kThreads && kSepSeg == kSepK / 2 &&
+                  kSepBlock % kThreads == 0,
+              "the matrix-core tile sum assumes 64x64 tiles, four waves, one segment per thread and one "
+              "segment value per k pair");
 
-__device__ __forceinline__ void load_params(const SepParams* __restrict__ par, int64_t i, double2& v0, double2& v1,
-                                            double2& v2, double2& v3, double2& v4, double2& v5) {
-  const double2* src = reinterpret_cast<const double2*>(par + i);
+// The axis record (q2, q1), (l1, c0), (cen, g) of beamlet i.
+__device__ __forceinline__ void load_axis(const SepParams* __restrict__ par, int64_t i, int axis, double2& v0,
+                                          double2& v1, double2& v2) {
+  const double2* src = reinterpret_cast<const double2*>(par + i) + 3 * axis;
   v0 = src[0];
   v1 = src[1];
   v2 = src[2];
-  v3 = src[3];
-  v4 = src[4];
-  v5 = src[5];
 }
 
-// One k pair of the complex product: (re, im) += Ey_k(m) Ex_k(n).
-__device__ __forceinline__ void sep_step(const float2 (&ex)[kSepK][kSepRow], const float2 (&ey)[kSepK][kSepRow], int k,
-                                         int m, int n, floatx16& accr, floatx16& acci) {
-  const float2 a = ey[k][m];
-  const float2 b = ex[k][n];
+// Evaluation state of one 16-pixel segment starting at pixel p0: the
+// log2-scaled envelope, and the phase as a 32-bit fixed-point fraction of a
+// revolution stepped by its exact second-order difference (wrapping integer
+// adds are exact modulo one revolution).  An invalid segment evaluates to zero.
+struct SegGen {
+  uint32_t th, d, dd;  // theta(j), theta(j + 1) - theta(j), 2 B   (2^-32 revolutions)
+  float t0, q2l, gl;
+};
+
+// r (revolutions, |r| < 2^52) modulo one, in 2^-32 units
+__device__ __forceinline__ uint32_t fixed_revolutions(double r) {
+  return (uint32_t)(int64_t)rint((r - rint(r)) * 4294967296.0);
+}
+
+__device__ __forceinline__ SegGen seg_setup(double2 v0, double2 v1, double2 v2, double p0, bool valid) {
+  const double q2 = v0.x, q1 = v0.y, l1 = v1.x, c0 = v1.y, cen = v2.x, g = v2.y;
+  // theta(p0 + j) = theta0 + j A + j^2 B  (revolutions)
+  const double x = fma(fma(q1, p0, l1), p0, c0);
+  const double hi = x * kInv2PiHi;
+  const double theta0 = (hi - rint(hi)) + fma(x, kInv2PiLo, fma(x, kInv2PiHi, -hi));
+  const uint32_t a = fixed_revolutions(fma(2.0 * q1, p0, l1) * kInv2PiHi);
+  const uint32_t b = fixed_revolutions(q1 * kInv2PiHi);
+  SegGen s;
+  s.th = fixed_revolutions(theta0);
+  s.d = a + b;
+  s.dd = 2u * b;
+  s.t0 = (float)(p0 - cen);
+  s.q2l = (float)(q2 * kLog2e);
+  s.gl = valid ? (float)(g * kLog2e) : -INFINITY;
+  return s;
+}
+
+// The segment's value at pixel p0 + j (j = 0, 1, ... in turn).
+__device__ __forceinline__ float2 seg_next(SegGen& s, int j) {
+  const float t = s.t0 + (float)j;
+  const float mag = __builtin_amdgcn_exp2f(fmaf(s.q2l * t, t, s.gl));
+  const float th = (float)(int32_t)s.th * 0x1p-32f;  // [-0.5, 0.5) revolutions
+  s.th += s.d;
+  s.d += s.dd;
+  return make_float2(mag * __builtin_amdgcn_cosf(th), mag * __builtin_amdgcn_sinf(th));
+}
+
+// One k pair of the complex product: (re, im) += a b with a = Ey_k(m),
+// b = Ex_k(n) (this lane's operands).
+__device__ __forceinline__ void sep_mfma(float2 a, float2 b, floatx16& accr, floatx16& acci) {
   accr = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.x, accr, 0, 0, 0);
   accr = __builtin_amdgcn_mfma_f32_32x32x2f32(-a.y, b.y, accr, 0, 0, 0);
   acci = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.y, acci, 0, 0, 0);
@@ -199,10 +239,9 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_sep_kernel(const tg_beam
                                                                    int64_t H, int64_t W, int32_t ntx, int32_t ntiles,
                                                                    int32_t nsplit, const int32_t* __restrict__ flags,
                                                                    float2* __restrict__ ws) {
-  __shared__ TileLists sh;
-  __shared__ float2 ex[kSepK][kSepRow];  // Ex_k(tx0 + n)
-  __shared__ float2 ey[kSepK][kSepRow];  // Ey_k(ty0 + m)
-  __shared__ SepParams spar[kSepK];
+  __shared__ int32_t list[kSepBlock];
+  __shared__ int32_t wave_cnt[kWaves];
+  __shared__ float2 fac[2][2][kSepK][kSepRow];  // [buffer][x: Ex_k(tx0 + n), y: Ey_k(ty0 + m)][k][pixel]
   if (!matrix_path(flags)) return;  // beamlet_sum_kernel covers this launch
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int tile = blockIdx.x % ntiles, split = blockIdx.x / ntiles;
@@ -210,97 +249,104 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_sep_kernel(const tg_beam
   const int64_t b_begin = n * split / nsplit, b_end = n * (split + 1) / nsplit;
   const float ftx0 = (float)tx0, fty0 = (float)ty0, ftx1 = (float)(tx0 + kTileW - 1), fty1 = (float)(ty0 + kTileH - 1);
   const int mb = (wave >> 1) * 32, nb = (wave & 1) * 32;
-  // this thread's segment: beamlet gk of the chunk, axis (waves 0-1: x, 2-3: y), pixels s0 .. s0 + 15
+  // this thread's segment: beamlet gk of a chunk, axis (waves 0-1: x, 2-3: y), pixels p0 .. p0 + 15
   const int gk = t & (kSepK - 1), gaxis = t >> 7, s0 = ((t >> 5) & 3) * kSepSeg;
+  const double p0 = (double)((gaxis == 0 ? tx0 : ty0) + s0);
   floatx16 accr, acci;
 #pragma unroll
   for (int j = 0; j < 16; ++j) accr[j] = acci[j] = 0.0f;
-  for (int64_t b0 = b_begin; b0 < b_end; b0 += kThreads) {
-    // beamlets of this chunk whose cutoff box meets the tile, in order
-    const int64_t b = b0 + t;
-    bool hit = false;
-    if (b < b_end) {
-      const tg_beamlet_coef& c = coef[b];
-      hit = c.cx + c.box_w >= ftx0 && c.cx - c.box_w <= ftx1 && c.cy + c.box_h >= fty0 && c.cy - c.box_h <= fty1;
-    }
-    const uint64_t mask = __ballot(hit);
-    if (lane == 0) sh.wave_cnt[wave] = __popcll(mask);
-    __syncthreads();
-    int off = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-      off += (w < wave) ? sh.wave_cnt[w] : 0;
-      total += sh.wave_cnt[w];
-    }
-    if (hit) sh.list[off + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)(b - b0);
-    __syncthreads();
-    // threads < kSepK: the next chunk's parameters (six 16-byte words, kept
-    // in named registers: an array here was placed in scratch), in flight
-    double2 n0, n1, n2, n3, n4, n5;
-    if (t < kSepK && t < total) load_params(par, b0 + sh.list[t], n0, n1, n2, n3, n4, n5);
-    for (int q0 = 0; q0 < total; q0 += kSepK) {
-      const int nk = min(kSepK, total - q0);
-      if (t < nk) {
-        double2* dst = reinterpret_cast<double2*>(&spar[t]);
-        dst[0] = n0;
-        dst[1] = n1;
-        dst[2] = n2;
-        dst[3] = n3;
-        dst[4] = n4;
-        dst[5] = n5;
+  for (int64_t blk = b_begin; blk < b_end; blk += kSepBlock) {
+    // the beamlets of [blk, blk + kSepBlock) whose cutoff box meets the tile, in order
+    int cnt = 0;
+    for (int64_t b0 = blk; b0 < b_end && b0 < blk + kSepBlock; b0 += kThreads) {
+      const int64_t b = b0 + t;
+      bool hit = false;
+      if (b < b_end) {
+        const tg_beamlet_coef& c = coef[b];
+        hit = c.cx + c.box_w >= ftx0 && c.cx - c.box_w <= ftx1 && c.cy + c.box_h >= fty0 && c.cy - c.box_h <= fty1;
       }
+      const uint64_t mask = __ballot(hit);
+      if (lane == 0) wave_cnt[wave] = __popcll(mask);
       __syncthreads();
-      if (t < kSepK && q0 + kSepK + t < total)
-        load_params(par, b0 + sh.list[q0 + kSepK + t], n0, n1, n2, n3, n4, n5);
-      {
-        float2* dst = gaxis == 0 ? &ex[gk][s0] : &ey[gk][s0];
-        if (gk < nk) {
-          // start at the segment's pixel nearest the factor's peak and step
-          // outwards both ways: magnitudes only decay, so an underflowed
-          // start or an overflowing ratio is never stepped through
-          const SepParams& P = spar[gk];
-          const double q2 = gaxis == 0 ? P.ar : P.cr, q1 = gaxis == 0 ? P.ai : P.ci;
-          const double l1 = gaxis == 0 ? P.di : P.ei, c0 = gaxis == 0 ? 0.0 : P.fi, g = gaxis == 0 ? 0.0 : P.g0;
-          const double cen = gaxis == 0 ? P.cx : P.cy;
-          const double p0 = (double)((gaxis == 0 ? tx0 : ty0) + s0);
-          const int js = (int)fmin(fmax(rint(cen - p0), 0.0), (double)(kSepSeg - 1));
-          const double ps = p0 + js, ds = ps - cen;
-          // Phi(p) = q2 (p - cen)^2 + g + i (q1 p^2 + l1 p + c0) along this axis
-          const float2 F = cexp_f(fma(q2 * ds, ds, g), fma(fma(q1, ps, l1), ps, c0));
-          float2 Rf = cexp_f(q2 * fma(2.0, ds, 1.0), fma(q1, fma(2.0, ps, 1.0), l1));    // Phi(p + 1) - Phi(p)
-          float2 Rb = cexp_f(-q2 * fma(2.0, ds, -1.0), -fma(q1, fma(2.0, ps, -1.0), l1));  // Phi(p - 1) - Phi(p)
-          const float2 K = gaxis == 0 ? make_float2(P.kx_re, P.kx_im) : make_float2(P.ky_re, P.ky_im);
-          dst[js] = F;
-          float2 Ff = F, Fb = F;
+      int off = cnt, total = 0;
 #pragma unroll
-          for (int s = 1; s < kSepSeg; ++s) {
-            if (js + s < kSepSeg) {
-              Ff = cmulf(Ff, Rf);
-              Rf = cmulf(Rf, K);
-              dst[js + s] = Ff;
-            }
-            if (js - s >= 0) {
-              Fb = cmulf(Fb, Rb);
-              Rb = cmulf(Rb, K);
-              dst[js - s] = Fb;
-            }
-          }
-        } else {
-#pragma unroll
-          for (int j = 0; j < kSepSeg; ++j) dst[j] = make_float2(0.0f, 0.0f);
-        }
+      for (int w = 0; w < kWaves; ++w) {
+        off += (w < wave) ? wave_cnt[w] : 0;
+        total += wave_cnt[w];
       }
-      __syncthreads();
+      if (hit) list[off + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)(b - blk);
+      cnt += total;
+      __syncthreads();  // wave_cnt is rewritten next; the list is complete after the last pass
+    }
+    if (cnt == 0) continue;
+    const int nch = (cnt + kSepK - 1) / kSepK;
+    // chunk q's beamlet for this thread (past cnt: the last one again, evaluated as zero)
+    auto beamlet = [&](int q) -> int64_t { return blk + list[min(q * kSepK + gk, cnt - 1)]; };
+    double2 v0, v1, v2;  // the parameters of the chunk evaluated next
+    load_axis(par, beamlet(0), gaxis, v0, v1, v2);
+    {
+      SegGen sg = seg_setup(v0, v1, v2, p0, gk < cnt);
+      float2* dst = &fac[0][gaxis][gk][s0];
+#pragma unroll
+      for (int j = 0; j < kSepSeg; ++j) dst[j] = seg_next(sg, j);
+    }
+    load_axis(par, beamlet(1), gaxis, v0, v1, v2);
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+      const int cur = c & 1;
       // A[m][k] = Ey_k(ty0 + mb + m) from lane m + 32 k, B[k][n] = Ex_k(tx0 + nb + n)
       // from lane n + 32 k (v_mfma_f32_32x32x2f32 operand layout); every
-      // chunk runs all kSepK (zero past nk), unrolled, so the LDS reads run
-      // ahead of the products
+      // chunk runs all kSepK (zero past cnt)
+      // (operands one k pair ahead, so LDS latency hides behind the products)
+      const float2* ea = &fac[cur][1][lane >> 5][mb + (lane & 31)];
+      const float2* eb = &fac[cur][0][lane >> 5][nb + (lane & 31)];
+      float2 a = ea[0], b = eb[0];
+      if (c + 1 < nch) {
+        // chunk c + 1 into the other buffer, one segment value per k pair,
+        // and chunk c + 2's parameters in flight
+        SegGen sg = seg_setup(v0, v1, v2, p0, (c + 1) * kSepK + gk < cnt);
+        load_axis(par, beamlet(c + 2), gaxis, v0, v1, v2);
+        // (the values are stored after the products: a store into the
+        // other buffer between them would order every later operand read
+        // behind it)
+        float2 val[kSepSeg];
 #pragma unroll
-      for (int kk = 0; kk < kSepK; kk += 2)
-        sep_step(ex, ey, kk + (lane >> 5), mb + (lane & 31), nb + (lane & 31), accr, acci);
-      __syncthreads();  // the next chunk rewrites ex, ey and the parameters
+        for (int kk = 0; kk < kSepK; kk += 2) {
+          float2 an = a, bn = b;
+          if (kk + 2 < kSepK) {
+            an = ea[(kk + 2) * kSepRow];
+            bn = eb[(kk + 2) * kSepRow];
+          }
+          sep_mfma(a, b, accr, acci);
+          val[kk >> 1] = seg_next(sg, kk >> 1);
+          a = an;
+          b = bn;
+          // issue order: products and the value's dependent vector
+          // instructions alternate (each covers the other's latency)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);  // three VALU
+          }
+        }
+        float2* dst = &fac[cur ^ 1][gaxis][gk][s0];
+#pragma unroll
+        for (int j = 0; j < kSepSeg; ++j) dst[j] = val[j];
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < kSepK; kk += 2) {
+          float2 an = a, bn = b;
+          if (kk + 2 < kSepK) {
+            an = ea[(kk + 2) * kSepRow];
+            bn = eb[(kk + 2) * kSepRow];
+          }
+          sep_mfma(a, b, accr, acci);
+          a = an;
+          b = bn;
+        }
+      }
+      __syncthreads();  // chunk c + 1 is complete; the next iteration rewrites buffer `cur`
     }
-    __syncthreads();  // the next chunk of beamlets rewrites the list
   }
   // accumulator j of lane l holds row 8 (j / 4) + 4 (l / 32) + j % 4, column l % 32
 #pragma unroll
