@@ -39,6 +39,7 @@ if ROOT not in sys.path:
 
 HBM_PEAK = 8.0e12  # B/s, MI355X
 SKIP_CPU = False
+BEAMLET_PATH = "tuned"  # --beamlet-path
 
 
 def _profiled(fname, key, cast=float):
@@ -642,7 +643,12 @@ def bench_beamlet(steps: int, warmup: int):
     coef0 = beamlet_coefficients(params, det, MODE_PROPAGATED, abcd, stride, cutoff=4.0)
     # the tile-sum path (matrix cores for separable beamlets, or the VALU
     # recurrence), then the beamlet slices under the chosen path
-    tuned_path, path_ms = autotune.tune_beamlet_path(coef0, n, (H, W))
+    if BEAMLET_PATH == "tuned":
+        tuned_path, path_ms = autotune.tune_beamlet_path(coef0, n, (H, W))
+    else:
+        tuned_path = autotune.BEAMLET_PATH_AUTO if BEAMLET_PATH == "auto" else autotune.BEAMLET_PATH_VALU
+        autotune.set_choice(autotune.BEAMLET_PATH, (n, H, W), tuned_path)
+        path_ms = {}
     tuned_splits, tune_ms = autotune.tune_beamlet_sum(coef0, n, (H, W))
     summer = BeamletSum(n, (H, W), dev)
     holder = {}
@@ -668,7 +674,11 @@ def bench_beamlet(steps: int, warmup: int):
         roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": 157.3, "unit": "TFLOP/s",
                 "frac": achieved / 157.3e12, "traffic": None, "flops_per_pair": 8, "tile_pairs": tile_pairs,
                 "timed_kernels": "tg_beamlet_sum (path flags + beamlet_sum_sep_kernel + slice sums), HIP events",
-                "peak_note": "MI355X dense FP32 matrix peak (v_mfma_f32_32x32x2f32)"}
+                "peak_note": "MI355X dense FP32 matrix peak (v_mfma_f32_32x32x2f32)",
+                "mfma_util_percent": _profiled("beamlet_sum_sep_kernel-beamlet_131072_1024sq.md",
+                                               "mfma_util_percent"),
+                "mfma_source": "profiles/beamlet_sum_sep_kernel-beamlet_131072_1024sq.md (MfmaUtil PMC); the "
+                               "factor evaluation's vector instructions do not overlap the f32 MFMAs on gfx950"}
     else:
         flops_per_eval = 14  # two complex f32 products (6 each) + complex accumulate (2)
         achieved = flops_per_eval * evaluated / (sum_ms * 1e-3)
@@ -823,9 +833,13 @@ def main():
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the CPU leg (for profiler runs; the JSON line then has cpu_baseline null)")
+    ap.add_argument("--beamlet-path", default="tuned", choices=["tuned", "auto", "valu"],
+                    help="beamlet workload: the tile-sum path (tuned: timed choice; auto: matrix cores for "
+                         "separable beamlets; valu: the recurrence kernel)")
     args = ap.parse_args()
-    global SKIP_CPU
+    global SKIP_CPU, BEAMLET_PATH
     SKIP_CPU = args.no_cpu_baseline
+    BEAMLET_PATH = args.beamlet_path
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path would shard by scan position / detector tile with no exchange)",
