@@ -187,7 +187,7 @@ def _grad_targets(input_ray, model, grad_vars):
             for f in dataclasses.fields(input_ray):
                 found.append(((input_ray, f.name), "ray", (f.name,)))
         elif isinstance(var, PathBuilder):
-            path = var._build()
+            path = var._path()
             root, keys = path[0], tuple(path[1:])
             if root is input_ray:
                 where = "ray"
@@ -199,7 +199,7 @@ def _grad_targets(input_ray, model, grad_vars):
         else:
             raise TypeError(f"run_with_grads: grad_vars holds {type(var).__name__}; use x.params.<field> paths")
         if not found:
-            what = var._build() if isinstance(var, PathBuilder) else var
+            what = var._path() if isinstance(var, PathBuilder) else var
             raise RuntimeError(f"Cannot find {what} in parameters")
         out += found
     return out

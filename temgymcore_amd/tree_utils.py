@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import dataclasses
 import numbers
-from typing import Any, Literal
+from typing import Any
 
 
 def _is_namedtuple(obj) -> bool:
@@ -22,48 +22,46 @@ def _is_namedtuple(obj) -> bool:
 
 
 class PathBuilder:
-    """One step of a parameter path: `key` of `parent`, fetched by attribute or item."""
+    """A parameter path: a root object and the steps taken from it, each an
+    attribute name or an item key.  Attribute access and indexing extend the
+    path; `_value()` walks it, `_path()` is (root, key, key, ...)."""
 
-    def __init__(self, parent: Any, key: int | str, getter: Literal["attr", "item"]):
-        self._parent = parent
-        self._original_key = key
-        self._getter = getter
-        true_parent = self._resolve_parent()
-        if not (dataclasses.is_dataclass(true_parent) or isinstance(true_parent, (list, tuple, dict))):
-            raise NotImplementedError(f"Unknown key type for cls {type(true_parent)}")
+    __slots__ = ("_origin", "_steps")
+
+    def __init__(self, origin: Any, steps: tuple = ()):
+        object.__setattr__(self, "_origin", origin)
+        object.__setattr__(self, "_steps", steps)
+        # the container the last step reads from must be one a path can enter
+        holder = _walk(origin, steps[:-1])
+        if steps and not (dataclasses.is_dataclass(holder) or isinstance(holder, (list, tuple, dict))):
+            raise NotImplementedError(f"a parameter path cannot enter {type(holder).__name__}")
 
     def __getattr__(self, name: str):
         if name.startswith("__"):
             raise AttributeError(name)
-        return type(self)(self, name, "attr")
+        return PathBuilder(self._origin, self._steps + ((True, name),))
 
-    def __getitem__(self, idx):
-        return type(self)(self, idx, "item")
+    def __getitem__(self, key):
+        return PathBuilder(self._origin, self._steps + ((False, key),))
 
-    def _resolve_root(self):
-        return self._parent._resolve_root() if isinstance(self._parent, PathBuilder) else self._parent
+    def _root(self):
+        return self._origin
 
-    def _resolve_parent(self):
-        return self._parent._resolve() if isinstance(self._parent, PathBuilder) else self._parent
+    def _value(self):
+        return _walk(self._origin, self._steps)
 
-    def _resolve(self):
-        get_from = self._resolve_parent()
-        if self._getter == "attr":
-            return getattr(get_from, self._original_key)
-        if self._getter == "item":
-            return get_from[self._original_key]
-        raise ValueError(f"Unknown get with {self._getter}")
-
-    def _build(self, children: tuple = (), original: bool = True) -> tuple:
-        """(root, key, key, ...) of this path."""
-        this = (self._original_key,) + tuple(children)
-        if not isinstance(self._parent, PathBuilder):
-            return (self._parent,) + this
-        return self._parent._build(this, original=original)
+    def _path(self) -> tuple:
+        return (self._origin,) + tuple(key for _, key in self._steps)
 
     def __repr__(self):
-        root, *keys = self._build()
-        return f"PathBuilder({type(root).__name__}, {tuple(keys)})"
+        return f"PathBuilder({type(self._origin).__name__}, {self._path()[1:]})"
+
+
+def _walk(obj, steps):
+    """The object reached from obj by (by_attribute, key) steps."""
+    for by_attr, key in steps:
+        obj = getattr(obj, key) if by_attr else obj[key]
+    return obj
 
 
 class HasParamsMixin:
@@ -75,7 +73,7 @@ class HasParamsMixin:
 
     @property
     def params(self):
-        builders = {f.name: PathBuilder(self, f.name, "attr") for f in dataclasses.fields(self) if f.init}
+        builders = {f.name: PathBuilder(self, ((True, f.name),)) for f in dataclasses.fields(self) if f.init}
         obj = object.__new__(type(self))
         for name, b in builders.items():
             object.__setattr__(obj, name, b)

@@ -195,7 +195,7 @@ def test_coefficients_from_dict_and_params_path():
     lens = AberratedLensKrivanek(z=0.1, focal_length=0.2, coeffs={"C30": 1.0})
     assert isinstance(lens.coeffs, KrivanekCoeffs) and lens.coeffs.C30 == 1.0
     path = lens.params.coeffs.C30
-    root, *keys = path._build()
+    root, *keys = path._path()
     assert root is lens and keys == ["coeffs", "C30"]
     with pytest.raises(TypeError):
         AberratedLensKrivanek(z=0.1, focal_length=0.2, coeffs=[1.0])

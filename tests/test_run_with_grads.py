@@ -120,9 +120,9 @@ def test_pathbuilder_paths():
     lens = Lens(z=0.5, focal_length=1.0)
     pb = lens.params.focal_length
     assert isinstance(pb, PathBuilder)
-    assert pb._build() == (lens, "focal_length")
-    assert pb._resolve() == 1.0
+    assert pb._path() == (lens, "focal_length")
+    assert pb._value() == 1.0
     det = Detector(z=1.0, pixel_size=(0.01, 0.02), shape=(8, 8))
-    assert det.params.pixel_size[1]._resolve() == 0.02
-    assert det.params.pixel_size[1]._build() == (det, "pixel_size", 1)
-    assert RAY.params.dx._resolve_root() is RAY
+    assert det.params.pixel_size[1]._value() == 0.02
+    assert det.params.pixel_size[1]._path() == (det, "pixel_size", 1)
+    assert RAY.params.dx._root() is RAY
