@@ -617,6 +617,13 @@ def _ellipse_pixel_count(coef_bytes, n, H, W):
     return total
 
 
+def _shape_name(shape):
+    from temgymcore_amd import autotune
+
+    w, h, run = autotune.TILE_SHAPES[int(shape)]
+    return f"{w}x{h} run {run}"
+
+
 def bench_beamlet(steps: int, warmup: int):
     import numpy as np
     import torch
@@ -650,6 +657,10 @@ def bench_beamlet(steps: int, warmup: int):
         autotune.set_choice(autotune.BEAMLET_PATH, (n, H, W), tuned_path)
         path_ms = {}
     tuned_splits, tune_ms = autotune.tune_beamlet_sum(coef0, n, (H, W))
+    tuned_shape, shape_ms = None, {}
+    if tuned_path == autotune.BEAMLET_PATH_VALU:
+        # the recurrence's tile shape (the matrix-core path has one shape)
+        tuned_shape, shape_ms = autotune.tune_beamlet_tile_shape(coef0, n, (H, W))
     summer = BeamletSum(n, (H, W), dev)
     holder = {}
 
@@ -705,7 +716,9 @@ def bench_beamlet(steps: int, warmup: int):
                                     "beamlet_sum_kernel (one exits at once) + split_sum_kernel",
                 "autotune": {"beamlet_path": {1: "auto", 2: "valu"}[tuned_path],
                              "path_candidate_ms": {{1: "auto", 2: "valu"}[k]: v for k, v in path_ms.items()},
-                             "beamlet_splits": tuned_splits, "candidate_ms": tune_ms}},
+                             "beamlet_splits": tuned_splits, "candidate_ms": tune_ms,
+                             "tile_shape": _shape_name(tuned_shape) if tuned_shape else "64x64 (matrix cores)",
+                             "tile_shape_ms": {_shape_name(k): v for k, v in shape_ms.items()}}},
         roofline=roof,
         cpu_baseline=cpu,
     )
@@ -754,7 +767,8 @@ def bench_gauss_fit(steps: int, warmup: int):
     from temgymcore_amd import autotune
 
     batch, params, target = gauss_fit.workload()
-    tuned_slice, tune_ms, batch = autotune.tune_gauss_fit(batch.grids, batch.counts, params, target)
+    tuned_slice, tune_ms, batch, tuned_shape, shape_ms = autotune.tune_gauss_fit(batch.grids, batch.counts, params,
+                                                                                 target)
     dev = batch.device
     loss = torch.empty(batch.n_inst, dtype=torch.float64, device=dev)
     grad = torch.empty((batch.n_gauss, 16), dtype=torch.float64, device=dev)
@@ -808,7 +822,9 @@ def bench_gauss_fit(steps: int, warmup: int):
                 "evaluated_pairs_per_pass": evaluated, "event_ms_per_step": kern_ms,
                 "graph_replay_ms_per_step": graph_ms, "forward_only_event_ms": fwd_kern_ms,
                 "kernels_per_step": "fit_coef, fit_forward, fit_loss, fit_loss_final, fit_moment, fit_grad",
-                "autotune": {"fit_slice": tuned_slice, "candidate_ms": tune_ms}},
+                "autotune": {"fit_slice": tuned_slice, "candidate_ms": tune_ms,
+                             "tile_shape": _shape_name(tuned_shape),
+                             "tile_shape_ms": {_shape_name(k): v for k, v in shape_ms.items()}}},
         roofline={"bound": "valu", "achieved": achieved / 1e12, "peak": 157.3, "unit": "TFLOP/s",
                   "frac": achieved / 157.3e12, "traffic": None,
                   "flops_per_eval": {"forward": flops_fwd, "backward": flops_bwd},

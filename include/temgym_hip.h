@@ -148,13 +148,26 @@ int32_t tg_device_cu_count(void);
  *   TG_TUNE_STEM_FIT       key {S, J, n_cam, 0}: tg_stem_fit launch variant
  *                          (tg_stem_fit_variant)
  *   TG_TUNE_BEAMLET_PATH   key {n, H, W, 0}: tg_beamlet_sum's tile sum
- *                          (tg_beamlet_path) */
+ *                          (tg_beamlet_path)
+ *   TG_TUNE_TILE_SHAPE     tile shape of the VALU recurrence tile sum
+ *                          (tg_tile_shape): key {n, H, W, 0} for
+ *                          tg_beamlet_sum, {n_inst, n_gauss, n_pix, 1} for
+ *                          tg_gauss_fit_plan (the plan records it) */
 enum tg_tune_knob {
   TG_TUNE_BEAMLET_SPLITS = 1,
   TG_TUNE_RAY_GRID = 2,
   TG_TUNE_FIT_SLICE = 3,
   TG_TUNE_STEM_FIT = 4,
-  TG_TUNE_BEAMLET_PATH = 5
+  TG_TUNE_BEAMLET_PATH = 5,
+  TG_TUNE_TILE_SHAPE = 6
+};
+/* Tile of the VALU recurrence tile sum (256 threads; each lane walks a run of
+ * pixels of one row): width x height, run length.  Default 64x64, run 16. */
+enum tg_tile_shape {
+  TG_TILE_64x64_RUN16 = 1,
+  TG_TILE_128x32_RUN16 = 2,
+  TG_TILE_32x128_RUN16 = 3,
+  TG_TILE_64x32_RUN8 = 4
 };
 /* TG_BEAMLET_PATH_AUTO: the matrix-core tile sum when every beamlet of the
  * launch is separable (no px*py term in Phi, envelope decaying along both
@@ -373,7 +386,7 @@ typedef struct tg_fit_plan {
   int64_t n_assigned;       /* rows that belong to an instance                    */
   int64_t n_pix;            /* length of the packed pixel arrays                  */
   float cut;
-  int32_t reserved;
+  int32_t tile_shape;       /* tg_tile_shape of the forward tile sum              */
   int64_t off_inst, off_items, off_loss_blocks, off_order, off_gauss_inst;
   int64_t off_coef, off_unit, off_partial, off_weight, off_loss_part, off_moments;
   int64_t table_bytes;      /* bytes of the tables uploaded by tg_gauss_fit_upload */

@@ -15,6 +15,8 @@ Knobs (include/temgym_hip.h, ``tg_tune_knob``):
                   per scan position and camera length)
   BEAMLET_PATH    tile sum of ``tg_beamlet_sum``: the matrix-core product for
                   separable beamlets (when all are), or the VALU recurrence
+  TILE_SHAPE      tile shape and run length of the VALU recurrence tile sum
+                  (``tg_beamlet_sum`` and the batched fits' forward pass)
 """
 from __future__ import annotations
 
@@ -25,11 +27,15 @@ import torch
 
 from . import _lib
 
-BEAMLET_SPLITS, RAY_GRID, FIT_SLICE, STEM_FIT, BEAMLET_PATH = 1, 2, 3, 4, 5
+BEAMLET_SPLITS, RAY_GRID, FIT_SLICE, STEM_FIT, BEAMLET_PATH, TILE_SHAPE = 1, 2, 3, 4, 5, 6
 _NAMES = {BEAMLET_SPLITS: "beamlet_splits", RAY_GRID: "ray_grid", FIT_SLICE: "fit_slice", STEM_FIT: "stem_fit",
-          BEAMLET_PATH: "beamlet_path"}
+          BEAMLET_PATH: "beamlet_path", TILE_SHAPE: "tile_shape"}
 STEM_FIT_SCAN, STEM_FIT_CAMERA = 1, 2
 BEAMLET_PATH_AUTO, BEAMLET_PATH_VALU = 1, 2
+# tg_tile_shape: (width, height, run length) of each tile shape
+TILE_64x64_RUN16, TILE_128x32_RUN16, TILE_32x128_RUN16, TILE_64x32_RUN8 = 1, 2, 3, 4
+TILE_SHAPES = {TILE_64x64_RUN16: (64, 64, 16), TILE_128x32_RUN16: (128, 32, 16), TILE_32x128_RUN16: (32, 128, 16),
+               TILE_64x32_RUN8: (64, 32, 8)}
 
 
 def _key(key):
@@ -86,6 +92,15 @@ def tune_beamlet_sum(coef: torch.Tensor, n: int, shape, candidates=(8, 16, 32, 6
     summer = BeamletSum(n, shape, coef.device)
     cands = [c for c in candidates if c <= max(n, 1)] or [1]
     return _pick(BEAMLET_SPLITS, (n, shape[0], shape[1]), cands, lambda: summer.run(coef), reps)
+
+
+def tune_beamlet_tile_shape(coef: torch.Tensor, n: int, shape, candidates=tuple(TILE_SHAPES), reps: int = 3):
+    """Tile shape of tg_beamlet_sum(n, H, W)'s VALU recurrence on these
+    coefficient records (matters only when that path runs)."""
+    from .gaussian import BeamletSum
+
+    summer = BeamletSum(n, shape, coef.device)
+    return _pick(TILE_SHAPE, (n, shape[0], shape[1]), list(candidates), lambda: summer.run(coef), reps)
 
 
 def tune_beamlet_path(coef: torch.Tensor, n: int, shape, candidates=(BEAMLET_PATH_AUTO, BEAMLET_PATH_VALU),
@@ -148,24 +163,38 @@ def tune_stem_fit(ws, params: torch.Tensor, targets: torch.Tensor, candidates=(S
 
 
 def tune_gauss_fit(grids, counts, params: torch.Tensor, target: torch.Tensor, candidates=(128, 256, 512, 1024),
-                   cutoff=None, reps: int = 3):
-    """Gaussians per forward slice for this batch geometry; returns
-    (best, timings, batch planned with the best choice)."""
+                   cutoff=None, reps: int = 3, shapes=tuple(TILE_SHAPES)):
+    """Gaussians per forward slice, then the forward tile shape, for this
+    batch geometry (each choice needs a new plan); returns
+    (best slice, slice timings, batch planned with both best choices,
+    best shape, shape timings)."""
     from .gauss_fit import GaussianFitBatch
 
     npix = sum(int(g.shape[0]) * int(g.shape[1]) for g in grids)
     key = (len(grids), int(sum(counts)), npix)
+    shape_key = key + (1,)
     loss = torch.empty(len(grids), dtype=torch.float64, device=params.device)
     grad = torch.empty((int(sum(counts)), 16), dtype=torch.float64, device=params.device)
+
+    def timed():
+        b = GaussianFitBatch(grids, counts, cutoff)
+        t = time_ms(lambda: b.run(params, target, None, loss, grad), reps)
+        del b
+        return t
+
     results = {}
     for c in candidates:
         set_choice(FIT_SLICE, key, c)
-        b = GaussianFitBatch(grids, counts, cutoff)
-        results[int(c)] = time_ms(lambda: b.run(params, target, None, loss, grad), reps)
-        del b
+        results[int(c)] = timed()
     best = min(results, key=results.get)
     set_choice(FIT_SLICE, key, best)
-    return best, results, GaussianFitBatch(grids, counts, cutoff)
+    shape_ms = {}
+    for sh in shapes:
+        set_choice(TILE_SHAPE, shape_key, sh)
+        shape_ms[int(sh)] = timed()
+    best_shape = min(shape_ms, key=shape_ms.get)
+    set_choice(TILE_SHAPE, shape_key, best_shape)
+    return best, results, GaussianFitBatch(grids, counts, cutoff), best_shape, shape_ms
 
 
 def save(path: str, entries) -> None:

@@ -115,6 +115,7 @@ __device__ __forceinline__ bool matrix_path(const int32_t* __restrict__ flags) {
   return !__syncthreads_or(bad);
 }
 
+template <int LPR, int RUN>
 __global__ __launch_bounds__(kThreads) void beamlet_sum_kernel(const tg_beamlet_coef* __restrict__ coef, int64_t n,
                                                                int64_t H, int64_t W, int32_t ntx, int32_t ntiles,
                                                                int32_t nsplit, const int32_t* __restrict__ flags,
@@ -122,21 +123,9 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_kernel(const tg_beamlet_
   __shared__ TileLists sh;
   if (matrix_path(flags)) return;  // beamlet_sum_sep_kernel covers this launch
   const int tile = blockIdx.x % ntiles, split = blockIdx.x / ntiles;
-  const int32_t tx0 = (tile % ntx) * kTileW, ty0 = (tile / ntx) * kTileH;
+  const int32_t tx0 = (tile % ntx) * (LPR * RUN), ty0 = (tile / ntx) * (kThreads / LPR);
   const int64_t b_begin = n * split / nsplit, b_end = n * (split + 1) / nsplit;
-  const int t = threadIdx.x;
-  const int32_t row = ty0 + t / kLanesPerRow;
-  const int32_t x0 = tx0 + (t % kLanesPerRow) * kRun;
-  float2 acc[kRun];
-#pragma unroll
-  for (int j = 0; j < kRun; ++j) acc[j] = make_float2(0.f, 0.f);
-  tile_sum(coef, b_begin, b_end, tx0, ty0, (int32_t)H, (int32_t)W, sh, acc);
-  if (row < H) {
-    float2* dst = ws + ((int64_t)split * H + row) * W;
-#pragma unroll
-    for (int j = 0; j < kRun; ++j)
-      if (x0 + j < W) dst[x0 + j] = acc[j];
-  }
+  tile_sum_store<LPR, RUN>(coef, b_begin, b_end, tx0, ty0, (int32_t)H, (int32_t)W, sh, ws + (int64_t)split * H * W);
 }
 
 // ---------------------------------------------------------------------------
@@ -504,6 +493,35 @@ int splits_for(int64_t n, int64_t H, int64_t W) {
   return (int)s;
 }
 
+// The recurrence tile sum (beamlet_sum_kernel) for a tile shape
+// (tg_tile_shape): one workgroup per (tile, beamlet slice).
+int launch_recurrence_sum(int32_t shape, const tg_beamlet_coef* coef, int64_t n, int64_t H, int64_t W, int nsplit,
+                          const int32_t* flags, float2* ws, hipStream_t s) {
+  const TileDims d = tile_dims(shape);
+  const int64_t ntx = tg::cdiv(W, (int64_t)d.w), ntiles = ntx * tg::cdiv(H, (int64_t)d.h);
+  TG_REQUIRE(ntiles * nsplit < (int64_t)1 << 31, "tg_beamlet_sum: detector too large");
+  const dim3 grid((unsigned)(ntiles * nsplit));
+  switch (shape) {
+    case TG_TILE_128x32_RUN16:
+      hipLaunchKernelGGL((beamlet_sum_kernel<8, 16>), grid, dim3(kThreads), 0, s, coef, n, H, W, (int32_t)ntx,
+                         (int32_t)ntiles, (int32_t)nsplit, flags, ws);
+      break;
+    case TG_TILE_32x128_RUN16:
+      hipLaunchKernelGGL((beamlet_sum_kernel<2, 16>), grid, dim3(kThreads), 0, s, coef, n, H, W, (int32_t)ntx,
+                         (int32_t)ntiles, (int32_t)nsplit, flags, ws);
+      break;
+    case TG_TILE_64x32_RUN8:
+      hipLaunchKernelGGL((beamlet_sum_kernel<8, 8>), grid, dim3(kThreads), 0, s, coef, n, H, W, (int32_t)ntx,
+                         (int32_t)ntiles, (int32_t)nsplit, flags, ws);
+      break;
+    default:
+      hipLaunchKernelGGL((beamlet_sum_kernel<kLanesPerRow, kRun>), grid, dim3(kThreads), 0, s, coef, n, H, W,
+                         (int32_t)ntx, (int32_t)ntiles, (int32_t)nsplit, flags, ws);
+  }
+  TG_LAUNCH_CHECK("beamlet_sum_kernel launch");
+  return TG_OK;
+}
+
 }  // namespace
 
 extern "C" int tg_beamlet_coefficients(const double* params, int64_t n, const double* abcd, int64_t abcd_stride,
@@ -551,6 +569,7 @@ extern "C" int tg_beamlet_sum(const tg_beamlet_coef* coef, int64_t n, int64_t he
   const int64_t need = sum_workspace_bytes(nsplit, n, height, width);
   TG_REQUIRE(workspace_bytes >= need, "tg_beamlet_sum: workspace %lld bytes < %lld", (long long)workspace_bytes,
              (long long)need);
+  int st = TG_OK;
   const int64_t ntx = tg::cdiv(width, kTileW), nty = tg::cdiv(height, kTileH);
   const int64_t ntiles = ntx * nty;
   TG_REQUIRE(ntiles * nsplit < (int64_t)1 << 31, "tg_beamlet_sum: detector too large");
@@ -568,10 +587,10 @@ extern "C" int tg_beamlet_sum(const tg_beamlet_coef* coef, int64_t n, int64_t he
                      (hipStream_t)stream, coef, (const SepParams*)par, n, height, width, (int32_t)ntx,
                      (int32_t)ntiles, (int32_t)nsplit, (const int32_t*)flags, (float2*)workspace);
   TG_LAUNCH_CHECK("beamlet_sum_sep_kernel launch");
-  hipLaunchKernelGGL(beamlet_sum_kernel, dim3((unsigned)(ntiles * nsplit)), dim3(kThreads), 0, (hipStream_t)stream,
-                     coef, n, height, width, (int32_t)ntx, (int32_t)ntiles, (int32_t)nsplit, (const int32_t*)flags,
-                     (float2*)workspace);
-  TG_LAUNCH_CHECK("beamlet_sum_kernel launch");
+  int32_t shape = TG_TILE_64x64_RUN16;
+  tg::tune_lookup(TG_TUNE_TILE_SHAPE, key, shape);
+  st = launch_recurrence_sum(shape, coef, n, height, width, nsplit, flags, (float2*)workspace, (hipStream_t)stream);
+  if (st != TG_OK) return st;
   const int64_t npix = height * width;
   hipLaunchKernelGGL(split_sum_kernel, dim3((unsigned)tg::cdiv(npix, 256)), dim3(256), 0, (hipStream_t)stream,
                      (const float2*)workspace, npix, (int32_t)nsplit, (float2*)field);

@@ -349,13 +349,14 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
   return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
 }
 
-// Pixels [xs, xe] of row `row` within the run [x0, x0 + kRun - 1] (and < W)
+// Pixels [xs, xe] of row `row` within the run [x0, x0 + RUN - 1] (and < W)
 // that lie inside the beamlet's cutoff ellipse; false when there are none.
 // Pixel indices are 32-bit (images are far below 2^31 pixels per side).
+template <int RUN = kRun>
 __device__ __forceinline__ bool row_extent(const tg_beamlet_coef& c, int32_t row, int32_t x0, int32_t W, int32_t& xs,
                                            int32_t& xe) {
   xs = x0;
-  xe = min(x0 + kRun - 1, W - 1);
+  xe = min(x0 + RUN - 1, W - 1);
   if (c.gamma > 0.0f) {
     const float v = (float)row - c.cy;
     const float t2 = fmaf(-c.gamma * v, v, c.beta);
@@ -363,7 +364,7 @@ __device__ __forceinline__ bool row_extent(const tg_beamlet_coef& c, int32_t row
     const float sq = __builtin_amdgcn_sqrtf(t2);
     const float mid = fmaf(-c.alpha, v, c.cx);
     // clamp in float first so the conversions stay in int32 range
-    const float lo = fminf(fmaxf(ceilf(mid - sq), (float)x0), (float)(x0 + kRun));
+    const float lo = fminf(fmaxf(ceilf(mid - sq), (float)x0), (float)(x0 + RUN));
     const float hi = fmaxf(fminf(floorf(mid + sq), (float)xe), (float)(x0 - 1));
     xs = (int32_t)lo;
     xe = (int32_t)hi;
@@ -390,18 +391,34 @@ struct TileLists {
   int32_t wave_cnt[kWaves];
 };
 
+// Tile shapes of the recurrence tile sum (tg_tile_shape): LPR lanes per row
+// of the tile, each walking RUN pixels, kThreads / LPR rows.
+struct TileDims {
+  int32_t w, h;
+};
+__host__ __device__ inline TileDims tile_dims(int32_t shape) {
+  switch (shape) {
+    case TG_TILE_128x32_RUN16: return {128, 32};
+    case TG_TILE_32x128_RUN16: return {32, 128};
+    case TG_TILE_64x32_RUN8: return {64, 32};
+    default: return {kTileW, kTileH};
+  }
+}
+
 // Adds the beamlets coef[b_begin, b_end) to the tile whose top-left pixel is
-// (ty0, tx0): lane t owns row ty0 + t / kLanesPerRow and the kRun pixels from
-// x0 = tx0 + (t % kLanesPerRow) kRun, accumulated into acc.  Beamlets whose
-// cutoff box misses the tile are skipped after a deterministic ballot
-// compaction; every thread of the block must call this.
+// (ty0, tx0): lane t owns row ty0 + t / LPR and the RUN pixels from
+// x0 = tx0 + (t % LPR) RUN, accumulated into acc.  Beamlets whose cutoff box
+// misses the tile are skipped after a deterministic ballot compaction; every
+// thread of the block must call this.
+template <int LPR, int RUN>
 __device__ __forceinline__ void tile_sum(const tg_beamlet_coef* __restrict__ coef, int64_t b_begin, int64_t b_end,
                                          int32_t tx0, int32_t ty0, int32_t H, int32_t W, TileLists& sh,
-                                         float2 (&acc)[kRun]) {
+                                         float2 (&acc)[RUN]) {
+  constexpr int TW = LPR * RUN, TH = kThreads / LPR;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int32_t row = ty0 + t / kLanesPerRow;
-  const int32_t x0 = tx0 + (t % kLanesPerRow) * kRun;
-  const float ftx0 = (float)tx0, fty0 = (float)ty0, ftx1 = (float)(tx0 + kTileW - 1), fty1 = (float)(ty0 + kTileH - 1);
+  const int32_t row = ty0 + t / LPR;
+  const int32_t x0 = tx0 + (t % LPR) * RUN;
+  const float ftx0 = (float)tx0, fty0 = (float)ty0, ftx1 = (float)(tx0 + TW - 1), fty1 = (float)(ty0 + TH - 1);
   for (int64_t b0 = b_begin; b0 < b_end; b0 += kThreads) {
     const int64_t b = b0 + t;
     bool hit = false;
@@ -425,12 +442,12 @@ __device__ __forceinline__ void tile_sum(const tg_beamlet_coef* __restrict__ coe
       const tg_beamlet_coef& c = coef[bi];
       if (row >= H) continue;
       int32_t xs, xe;
-      if (!row_extent(c, row, x0, W, xs, xe)) continue;
+      if (!row_extent<RUN>(c, row, x0, W, xs, xe)) continue;
       float2 F, R;
       run_start(c, xs, row, F, R);
       const float2 K = make_float2(c.k_re, c.k_im);
 #pragma unroll
-      for (int j = 0; j < kRun; ++j) {
+      for (int j = 0; j < RUN; ++j) {
         const int32_t pxj = x0 + j;
         if (pxj >= xs && pxj <= xe) {
           acc[j].x += F.x;
@@ -441,6 +458,27 @@ __device__ __forceinline__ void tile_sum(const tg_beamlet_coef* __restrict__ coe
       }
     }
     __syncthreads();  // the list is rewritten by the next chunk
+  }
+}
+
+// One tile of a recurrence tile sum, stored into the rows of `dst_rows`
+// (row r of the image at dst_rows + r * W): the body of the tile-sum kernels.
+template <int LPR, int RUN>
+__device__ __forceinline__ void tile_sum_store(const tg_beamlet_coef* __restrict__ coef, int64_t b_begin,
+                                               int64_t b_end, int32_t tx0, int32_t ty0, int32_t H, int32_t W,
+                                               TileLists& sh, float2* __restrict__ dst_rows) {
+  const int t = threadIdx.x;
+  const int32_t row = ty0 + t / LPR;
+  const int32_t x0 = tx0 + (t % LPR) * RUN;
+  float2 acc[RUN];
+#pragma unroll
+  for (int j = 0; j < RUN; ++j) acc[j] = make_float2(0.f, 0.f);
+  tile_sum<LPR, RUN>(coef, b_begin, b_end, tx0, ty0, H, W, sh, acc);
+  if (row < H) {
+    float2* dst = dst_rows + (int64_t)row * W;
+#pragma unroll
+    for (int j = 0; j < RUN; ++j)
+      if (x0 + j < W) dst[x0 + j] = acc[j];
   }
 }
 

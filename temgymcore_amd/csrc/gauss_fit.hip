@@ -79,6 +79,7 @@ __global__ __launch_bounds__(256) void fit_coef_kernel(const double* __restrict_
   coef[g] = o;
 }
 
+template <int LPR, int RUN>
 __global__ __launch_bounds__(kThreads) void fit_forward_kernel(const tg_beamlet_coef* __restrict__ coef,
                                                                const FitInst* __restrict__ insts,
                                                                const FitItem* __restrict__ items,
@@ -87,21 +88,10 @@ __global__ __launch_bounds__(kThreads) void fit_forward_kernel(const tg_beamlet_
   const FitItem it = items[blockIdx.x];
   const FitInst& I = insts[it.inst];
   const int32_t H = I.H, W = I.W;
-  const int32_t tx0 = (it.tile % I.ntx) * kTileW, ty0 = (it.tile / I.ntx) * kTileH;
+  const int32_t tx0 = (it.tile % I.ntx) * (LPR * RUN), ty0 = (it.tile / I.ntx) * (kThreads / LPR);
   const int64_t b_begin = I.g0 + I.n * it.slice / I.nslice, b_end = I.g0 + I.n * (it.slice + 1) / I.nslice;
-  const int t = threadIdx.x;
-  const int32_t row = ty0 + t / kLanesPerRow;
-  const int32_t x0 = tx0 + (t % kLanesPerRow) * kRun;
-  float2 acc[kRun];
-#pragma unroll
-  for (int j = 0; j < kRun; ++j) acc[j] = make_float2(0.f, 0.f);
-  tile_sum(coef, b_begin, b_end, tx0, ty0, H, W, sh, acc);
-  if (row < H) {
-    float2* dst = partial + I.ws0 + ((int64_t)it.slice * H + row) * (int64_t)W;
-#pragma unroll
-    for (int j = 0; j < kRun; ++j)
-      if (x0 + j < W) dst[x0 + j] = acc[j];
-  }
+  tile_sum_store<LPR, RUN>(coef, b_begin, b_end, tx0, ty0, H, W, sh,
+                           partial + I.ws0 + (int64_t)it.slice * H * (int64_t)W);
 }
 
 __global__ __launch_bounds__(256) void fit_loss_kernel(const FitInst* __restrict__ insts,
@@ -311,6 +301,12 @@ int build(const tg_fit_instance* inst, int32_t n_inst, int64_t n_gauss, double n
     int32_t tuned = 0;
     if (tg::tune_lookup(TG_TUNE_FIT_SLICE, key, tuned)) slice_gauss = std::min<int64_t>(std::max(tuned, 16), 4096);
   }
+  int32_t shape = TG_TILE_64x64_RUN16;
+  {
+    const int64_t key[4] = {n_inst, n_gauss, n_pix, 1};
+    tg::tune_lookup(TG_TUNE_TILE_SHAPE, key, shape);
+  }
+  const TileDims tile = tile_dims(shape);
   Tables t;
   t.inst.resize(n_inst);
   t.gauss_inst.assign(n_gauss, -1);
@@ -325,11 +321,11 @@ int build(const tg_fit_instance* inst, int32_t n_inst, int64_t n_gauss, double n
     for (int k = 0; k < 6; ++k) f.m[k] = s.pix2m[k];
     f.H = s.height;
     f.W = s.width;
-    f.ntx = (int32_t)tg::cdiv(s.width, kTileW);
+    f.ntx = (int32_t)tg::cdiv(s.width, tile.w);
     f.nslice = slices_for(s.n_gauss, slice_gauss);
     const int64_t npix = (int64_t)s.height * s.width;
     ws_elems += (int64_t)f.nslice * npix;
-    n_items += tg::cdiv(s.height, kTileH) * f.ntx * f.nslice;
+    n_items += tg::cdiv(s.height, tile.h) * f.ntx * f.nslice;
     f.lb0 = (int32_t)n_blocks;
     n_blocks += tg::cdiv(npix, kLossChunk);
     f.lb1 = (int32_t)n_blocks;
@@ -348,7 +344,7 @@ int build(const tg_fit_instance* inst, int32_t n_inst, int64_t n_gauss, double n
   t.items.reserve(n_items);
   for (int32_t i : by_cost) {
     const FitInst& f = t.inst[i];
-    const int32_t ntiles = (int32_t)(tg::cdiv(f.H, kTileH) * f.ntx);
+    const int32_t ntiles = (int32_t)(tg::cdiv(f.H, tile.h) * f.ntx);
     for (int32_t tile = 0; tile < ntiles; ++tile)
       for (int32_t sl = 0; sl < f.nslice; ++sl) t.items.push_back(FitItem{i, tile, sl, 0});
   }
@@ -374,6 +370,7 @@ int build(const tg_fit_instance* inst, int32_t n_inst, int64_t n_gauss, double n
   p.n_assigned = n_assigned;
   p.n_pix = n_pix;
   p.cut = nsigma > 0.0 ? (float)(nsigma * nsigma) : 80.0f;
+  p.tile_shape = shape;
   int64_t off = 0;
   auto place = [&](int64_t bytes) {
     const int64_t o = off;
@@ -415,8 +412,8 @@ extern "C" int tg_gauss_fit_upload(const tg_fit_plan* plan, const tg_fit_instanc
   const int rc = build(inst, plan->n_inst, plan->n_gauss, 0.0, &check, &t);
   if (rc != TG_OK) return rc;
   TG_REQUIRE(check.table_bytes == plan->table_bytes && check.workspace_bytes == plan->workspace_bytes &&
-                 check.n_items == plan->n_items && check.n_pix == plan->n_pix,
-             "tg_gauss_fit_upload: instances (or the tuned slicing) differ from the plan's");
+                 check.n_items == plan->n_items && check.n_pix == plan->n_pix && check.tile_shape == plan->tile_shape,
+             "tg_gauss_fit_upload: instances (or the tuned slicing or tile shape) differ from the plan's");
   std::vector<char> host((size_t)plan->table_bytes, 0);
   auto put = [&](int64_t off, const void* src, size_t bytes) {
     if (bytes) memcpy(host.data() + off, src, bytes);
@@ -439,6 +436,8 @@ extern "C" int tg_gauss_fit_run(const tg_fit_plan* plan, void* workspace, const 
   tg::clear_error();
   TG_REQUIRE(plan != nullptr && workspace != nullptr, "tg_gauss_fit_run: NULL plan or workspace");
   TG_REQUIRE(plan->n_gauss == 0 || params != nullptr, "tg_gauss_fit_run: NULL params");
+  TG_REQUIRE(plan->tile_shape >= TG_TILE_64x64_RUN16 && plan->tile_shape <= TG_TILE_64x32_RUN8,
+             "tg_gauss_fit_run: plan has an unknown tile shape %d", plan->tile_shape);
   TG_REQUIRE(target != nullptr || (loss == nullptr && grad == nullptr),
              "tg_gauss_fit_run: loss and grad need target intensities");
   TG_REQUIRE(reinterpret_cast<uintptr_t>(workspace) % 256 == 0, "tg_gauss_fit_run: workspace not 256-byte aligned");
@@ -461,8 +460,21 @@ extern "C" int tg_gauss_fit_run(const tg_fit_plan* plan, void* workspace, const 
                        plan->cut, coef, unit);
     TG_LAUNCH_CHECK("fit_coef_kernel launch");
   }
-  hipLaunchKernelGGL(fit_forward_kernel, dim3((unsigned)plan->n_items), dim3(kThreads), 0, s, coef, insts, items,
-                     partial);
+  const dim3 fgrid((unsigned)plan->n_items);
+  switch (plan->tile_shape) {
+    case TG_TILE_128x32_RUN16:
+      hipLaunchKernelGGL((fit_forward_kernel<8, 16>), fgrid, dim3(kThreads), 0, s, coef, insts, items, partial);
+      break;
+    case TG_TILE_32x128_RUN16:
+      hipLaunchKernelGGL((fit_forward_kernel<2, 16>), fgrid, dim3(kThreads), 0, s, coef, insts, items, partial);
+      break;
+    case TG_TILE_64x32_RUN8:
+      hipLaunchKernelGGL((fit_forward_kernel<8, 8>), fgrid, dim3(kThreads), 0, s, coef, insts, items, partial);
+      break;
+    default:
+      hipLaunchKernelGGL((fit_forward_kernel<kLanesPerRow, kRun>), fgrid, dim3(kThreads), 0, s, coef, insts, items,
+                         partial);
+  }
   TG_LAUNCH_CHECK("fit_forward_kernel launch");
   hipLaunchKernelGGL(fit_loss_kernel, dim3((unsigned)plan->n_loss_blocks), dim3(256), 0, s, insts, blocks, partial,
                      target, (float2*)field, weight, loss_part);

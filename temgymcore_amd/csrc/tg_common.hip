@@ -73,9 +73,12 @@ int32_t tg_device_cu_count(void) { return tg::cu_count(); }
 
 int tg_tune_set(int32_t knob, const int64_t* key, int32_t value) {
   tg::clear_error();
-  TG_REQUIRE(knob >= TG_TUNE_BEAMLET_SPLITS && knob <= TG_TUNE_BEAMLET_PATH, "tg_tune_set: unknown knob %d", knob);
+  TG_REQUIRE(knob >= TG_TUNE_BEAMLET_SPLITS && knob <= TG_TUNE_TILE_SHAPE, "tg_tune_set: unknown knob %d", knob);
   TG_REQUIRE(key != nullptr, "tg_tune_set: NULL key");
   TG_REQUIRE(value > 0, "tg_tune_set: value must be positive");
+  TG_REQUIRE(knob != TG_TUNE_TILE_SHAPE || value <= TG_TILE_64x32_RUN8, "tg_tune_set: unknown tile shape %d", value);
+  TG_REQUIRE(knob != TG_TUNE_BEAMLET_PATH || value <= TG_BEAMLET_PATH_VALU, "tg_tune_set: unknown tile-sum path %d",
+             value);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune[TuneKey{knob, current_device(), key[0], key[1], key[2], key[3]}] = value;
   return TG_OK;

@@ -41,7 +41,7 @@ class FitInstance(ctypes.Structure):
 class FitPlan(ctypes.Structure):
     _fields_ = [("n_inst", ctypes.c_int32), ("n_items", ctypes.c_int32), ("n_loss_blocks", ctypes.c_int32),
                 ("n_moment_blocks", ctypes.c_int32), ("n_gauss", ctypes.c_int64), ("n_assigned", ctypes.c_int64),
-                ("n_pix", ctypes.c_int64), ("cut", ctypes.c_float), ("reserved", ctypes.c_int32)] + [
+                ("n_pix", ctypes.c_int64), ("cut", ctypes.c_float), ("tile_shape", ctypes.c_int32)] + [
         (name, ctypes.c_int64) for name in (
             "off_inst", "off_items", "off_loss_blocks", "off_order", "off_gauss_inst", "off_coef", "off_unit",
             "off_partial", "off_weight", "off_loss_part", "off_moments", "table_bytes", "workspace_bytes")]

@@ -53,6 +53,43 @@ def test_beamlet_path_knob_is_accepted():
     assert autotune.get_choice(autotune.BEAMLET_PATH, key) is None
 
 
+def test_tile_shape_knob_checks_values():
+    key = (1000, 64, 64)
+    for shape in autotune.TILE_SHAPES:
+        autotune.set_choice(autotune.TILE_SHAPE, key, shape)
+        assert autotune.get_choice(autotune.TILE_SHAPE, key) == shape
+    with pytest.raises(ValueError):
+        autotune.set_choice(autotune.TILE_SHAPE, key, len(autotune.TILE_SHAPES) + 1)
+    with pytest.raises(ValueError):
+        autotune.set_choice(autotune.BEAMLET_PATH, key, 3)
+
+
+def test_fit_plan_records_tuned_tile_shape():
+    """The plan's forward items tile each instance with the tuned shape."""
+    lib = _lib.load()
+    arr = (FitInstance * 2)()
+    dims = [(150, 130), (40, 300)]  # (height, width)
+    for i, (h, w) in enumerate(dims):
+        arr[i].gauss_begin, arr[i].n_gauss, arr[i].pix_begin = 10 * i, 10, sum(a * b for a, b in dims[:i])
+        arr[i].height, arr[i].width = h, w
+        arr[i].pix2m[:] = [0.0, -1e-6, 0.0, 1e-6, 0.0, 0.0]
+    npix = sum(a * b for a, b in dims)
+
+    def plan():
+        p = FitPlan()
+        assert lib.tg_gauss_fit_plan(arr, 2, 20, 0.0, ctypes.byref(p)) == 0
+        return p
+
+    p = plan()
+    assert p.tile_shape == autotune.TILE_64x64_RUN16
+    for shape, (tw, th, _) in autotune.TILE_SHAPES.items():
+        autotune.set_choice(autotune.TILE_SHAPE, (2, 20, npix, 1), shape)
+        p = plan()
+        assert p.tile_shape == shape
+        tiles = sum(-(-h // th) * -(-w // tw) for h, w in dims)
+        assert p.n_items == tiles  # one slice per instance (10 Gaussians)
+
+
 def test_fit_plan_follows_tuned_slice():
     lib = _lib.load()
     arr = (FitInstance * 1)()
