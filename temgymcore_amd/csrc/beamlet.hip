@@ -131,8 +131,8 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_kernel(const tg_beamlet_
 // ---------------------------------------------------------------------------
 // Separable beamlets on the matrix cores.  A tile is the complex matrix
 // product  T[m][n] = sum_k Ey_k(ty0 + m) Ex_k(tx0 + n)  over the beamlets whose
-// cutoff box meets it: four real f32 MFMA products per k (re re, -im im,
-// re im, im re).  Wave w owns the 32x32 block at rows 32 (w / 2), columns
+// cutoff box meets it: three real f32 MFMA products per k (sep_mfma).
+// Wave w owns the 32x32 block at rows 32 (w / 2), columns
 // 32 (w % 2) of the 64x64 tile.  The factors of kSepK beamlets at a time are
 // staged in LDS, double-buffered: a wave evaluates chunk c + 1 into one
 // buffer while issuing chunk c's products from the other, one barrier per
@@ -143,12 +143,12 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_kernel(const tg_beamlet_
 //   exp2(q2' (p - cen)^2 + g') (cos, sin)(2 pi theta(p)),
 //   theta(p0 + j) = theta0 + j A + j^2 B   (revolutions),
 // theta0 being the segment start's phase reduced in double-double.  theta0,
-// A and B are held modulo one revolution as 32-bit fixed-point fractions and
-// theta is stepped by its exact second difference with wrapping integer
-// adds, so a value's phase is good to ~1e-7 revolutions whatever the size
-// of the linear and quadratic phase terms (an f32 sum lost 1e-5 revolutions
-// on strongly curved input-plane beamlets).  Parameters are loaded one chunk
-// ahead of their use.
+// A and B are reduced modulo one revolution as 32-bit fixed-point fractions
+// (exact whatever the size of the linear and quadratic phase terms; an f32
+// sum theta0 + jA + j^2 B lost 1e-5 revolutions on strongly curved
+// input-plane beamlets), and seed a unit-phasor recurrence (two transcendental
+// pairs per segment instead of a sine and cosine per value).  Parameters are
+// loaded one chunk ahead of their use.
 // ---------------------------------------------------------------------------
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
@@ -173,17 +173,27 @@ __device__ __forceinline__ void load_axis(const SepParams* __restrict__ par, int
 }
 
 // Evaluation state of one 16-pixel segment starting at pixel p0: the
-// log2-scaled envelope, and the phase as a 32-bit fixed-point fraction of a
-// revolution stepped by its exact second-order difference (wrapping integer
-// adds are exact modulo one revolution).  An invalid segment evaluates to zero.
+// log2-scaled envelope, evaluated directly, and the unit phasor
+// z_j = exp(2 pi i theta(p0 + j)) stepped by the unit ratios
+// r_j = z_{j+1} / z_j, r_{j+1} = r_j k (|z| = 1: nothing to underflow).  The
+// seeds come from theta0, A and B reduced modulo one revolution as 32-bit
+// fixed-point fractions (exact wrapping integer arithmetic), so the phase
+// error is that of 16 f32 rotations (CPU emulation of this scheme: worst
+// 1.4e-5 rad, |z| within 6e-6 of 1).  An invalid segment evaluates to zero.
 struct SegGen {
-  uint32_t th, d, dd;  // theta(j), theta(j + 1) - theta(j), 2 B   (2^-32 revolutions)
+  float2 z, r, k;
   float t0, q2l, gl;
 };
 
 // r (revolutions, |r| < 2^52) modulo one, in 2^-32 units
 __device__ __forceinline__ uint32_t fixed_revolutions(double r) {
   return (uint32_t)(int64_t)rint((r - rint(r)) * 4294967296.0);
+}
+
+// exp(2 pi i u 2^-32)
+__device__ __forceinline__ float2 phasor(uint32_t u) {
+  const float t = (float)(int32_t)u * 0x1p-32f;  // [-0.5, 0.5) revolutions
+  return make_float2(__builtin_amdgcn_cosf(t), __builtin_amdgcn_sinf(t));
 }
 
 __device__ __forceinline__ SegGen seg_setup(double2 v0, double2 v1, double2 v2, double p0, bool valid) {
@@ -195,9 +205,9 @@ __device__ __forceinline__ SegGen seg_setup(double2 v0, double2 v1, double2 v2, 
   const uint32_t a = fixed_revolutions(fma(2.0 * q1, p0, l1) * kInv2PiHi);
   const uint32_t b = fixed_revolutions(q1 * kInv2PiHi);
   SegGen s;
-  s.th = fixed_revolutions(theta0);
-  s.d = a + b;
-  s.dd = 2u * b;
+  s.z = phasor(fixed_revolutions(theta0));
+  s.r = phasor(a + b);  // theta(1) - theta(0) = A + B
+  s.k = phasor(2u * b);  // second difference 2 B
   s.t0 = (float)(p0 - cen);
   s.q2l = (float)(q2 * kLog2e);
   s.gl = valid ? (float)(g * kLog2e) : -INFINITY;
@@ -208,19 +218,22 @@ __device__ __forceinline__ SegGen seg_setup(double2 v0, double2 v1, double2 v2, 
 __device__ __forceinline__ float2 seg_next(SegGen& s, int j) {
   const float t = s.t0 + (float)j;
   const float mag = __builtin_amdgcn_exp2f(fmaf(s.q2l * t, t, s.gl));
-  const float th = (float)(int32_t)s.th * 0x1p-32f;  // [-0.5, 0.5) revolutions
-  s.th += s.d;
-  s.d += s.dd;
-  return make_float2(mag * __builtin_amdgcn_cosf(th), mag * __builtin_amdgcn_sinf(th));
+  const float2 v = make_float2(mag * s.z.x, mag * s.z.y);
+  s.z = cmulf(s.z, s.r);
+  s.r = cmulf(s.r, s.k);
+  return v;
 }
 
-// One k pair of the complex product: (re, im) += a b with a = Ey_k(m),
-// b = Ex_k(n) (this lane's operands).
-__device__ __forceinline__ void sep_mfma(float2 a, float2 b, floatx16& accr, floatx16& acci) {
-  accr = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.x, accr, 0, 0, 0);
-  accr = __builtin_amdgcn_mfma_f32_32x32x2f32(-a.y, b.y, accr, 0, 0, 0);
-  acci = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.y, acci, 0, 0, 0);
-  acci = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.x, acci, 0, 0, 0);
+// One k pair of the complex product a b (a = Ey_k(m), b = Ex_k(n): this
+// lane's operands) in three real products (Gauss): p1 += a.re b.re,
+// p2 += a.im b.im, p3 += (a.re + a.im)(b.re + b.im); then Re = p1 - p2 and
+// Im = p3 - (p1 + p2).  Three MFMAs per k pair instead of four (no vector
+// instruction and MFMA of one SIMD run together, so the two adds cost far
+// less than the fourth product).
+__device__ __forceinline__ void sep_mfma(float2 a, float2 b, floatx16& p1, floatx16& p2, floatx16& p3) {
+  p1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.x, p1, 0, 0, 0);
+  p2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.y, p2, 0, 0, 0);
+  p3 = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x + a.y, b.x + b.y, p3, 0, 0, 0);
 }
 
 __global__ __launch_bounds__(kThreads) void beamlet_sum_sep_kernel(const tg_beamlet_coef* __restrict__ coef,
@@ -241,9 +254,9 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_sep_kernel(const tg_beam
   // this thread's segment: beamlet gk of a chunk, axis (waves 0-1: x, 2-3: y), pixels p0 .. p0 + 15
   const int gk = t & (kSepK - 1), gaxis = t >> 7, s0 = ((t >> 5) & 3) * kSepSeg;
   const double p0 = (double)((gaxis == 0 ? tx0 : ty0) + s0);
-  floatx16 accr, acci;
+  floatx16 p1, p2, p3;
 #pragma unroll
-  for (int j = 0; j < 16; ++j) accr[j] = acci[j] = 0.0f;
+  for (int j = 0; j < 16; ++j) p1[j] = p2[j] = p3[j] = 0.0f;
   for (int64_t blk = b_begin; blk < b_end; blk += kSepBlock) {
     // the beamlets of [blk, blk + kSepBlock) whose cutoff box meets the tile, in order
     int cnt = 0;
@@ -295,6 +308,9 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_sep_kernel(const tg_beam
         // and chunk c + 2's parameters in flight
         SegGen sg = seg_setup(v0, v1, v2, p0, (c + 1) * kSepK + gk < cnt);
         load_axis(par, beamlet(c + 2), gaxis, v0, v1, v2);
+        // keep these loads here: sunk below the products (as the scheduler
+        // does by itself), the next chunk waited a whole memory round trip
+        __builtin_amdgcn_sched_barrier(0);
         // (the values are stored after the products: a store into the
         // other buffer between them would order every later operand read
         // behind it)
@@ -306,16 +322,16 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_sep_kernel(const tg_beam
             an = ea[(kk + 2) * kSepRow];
             bn = eb[(kk + 2) * kSepRow];
           }
-          sep_mfma(a, b, accr, acci);
+          sep_mfma(a, b, p1, p2, p3);
           val[kk >> 1] = seg_next(sg, kk >> 1);
           a = an;
           b = bn;
           // issue order: products and the value's dependent vector
           // instructions alternate (each covers the other's latency)
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
+          for (int i = 0; i < 3; ++i) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
-            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);  // three VALU
+            __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // four VALU
           }
         }
         float2* dst = &fac[cur ^ 1][gaxis][gk][s0];
@@ -329,7 +345,7 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_sep_kernel(const tg_beam
             an = ea[(kk + 2) * kSepRow];
             bn = eb[(kk + 2) * kSepRow];
           }
-          sep_mfma(a, b, accr, acci);
+          sep_mfma(a, b, p1, p2, p3);
           a = an;
           b = bn;
         }
@@ -342,7 +358,7 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_sep_kernel(const tg_beam
   for (int j = 0; j < 16; ++j) {
     const int64_t py = ty0 + mb + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
     const int64_t px = tx0 + nb + (lane & 31);
-    if (py < H && px < W) ws[((int64_t)split * H + py) * W + px] = make_float2(accr[j], acci[j]);
+    if (py < H && px < W) ws[((int64_t)split * H + py) * W + px] = make_float2(p1[j] - p2[j], p3[j] - (p1[j] + p2[j]));
   }
 }
 
