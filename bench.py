@@ -739,21 +739,27 @@ def _all_separable(coef_bytes):
 
 
 def _tile_pair_count(coef_bytes, n, H, W, tile=64):
-    """(beamlet, pixel) pairs of the 64x64 tiles each beamlet's cutoff box
-    meets (the box test of the tile sums), 4096 per tile."""
+    """(beamlet, pixel) pairs of the 64x64 tiles each separable beamlet's
+    cutoff ellipse meets (the matrix-core tile sum's list test: the tile's
+    pixel centre nearest the centre inside u^2 + gamma v^2 <= beta, in f32),
+    4096 per tile."""
     import numpy as np
 
     rec = np.frombuffer(coef_bytes, dtype=np.dtype(_COEF_DTYPE))
-    cx, cy = rec["c"][:, 0].astype(np.float64), rec["c"][:, 1].astype(np.float64)
-    bw, bh = rec["box"][:, 0].astype(np.float64), rec["box"][:, 1].astype(np.float64)
-    ntx, nty = -(-W // tile), -(-H // tile)
-    x0 = np.clip(np.ceil((cx - bw - (tile - 1)) / tile), 0, ntx)
-    x1 = np.clip(np.floor((cx + bw) / tile), -1, ntx - 1)
-    y0 = np.clip(np.ceil((cy - bh - (tile - 1)) / tile), 0, nty)
-    y1 = np.clip(np.floor((cy + bh) / tile), -1, nty - 1)
-    nx = np.maximum(x1 - x0 + 1, 0)
-    ny = np.maximum(y1 - y0 + 1, 0)
-    return int((nx * ny).sum()) * tile * tile
+    f32 = np.float32
+    cx, cy = rec["c"][:, 0].astype(f32), rec["c"][:, 1].astype(f32)
+    gamma, beta = rec["abg"][:, 2].astype(f32), rec["abg"][:, 1].astype(f32)
+    tx0 = (np.arange(-(-W // tile)) * tile).astype(f32)
+    ty0 = (np.arange(-(-H // tile)) * tile).astype(f32)
+    total = 0
+    for lo in range(0, n, 4096):
+        sl = slice(lo, min(n, lo + 4096))
+        du = np.maximum(np.maximum(tx0[None, :] - cx[sl, None], cx[sl, None] - (tx0[None, :] + f32(tile - 1))), f32(0))
+        dv = np.maximum(np.maximum(ty0[None, :] - cy[sl, None], cy[sl, None] - (ty0[None, :] + f32(tile - 1))), f32(0))
+        gv = (gamma[sl, None] * dv).astype(np.float64) * dv  # fma(gamma dv, dv, du^2) in one rounding
+        q = (gv[:, :, None] + (du * du).astype(np.float64)[:, None, :]).astype(f32)
+        total += int((q <= beta[sl, None, None]).sum())
+    return total * tile * tile
 
 
 def bench_gauss_fit(steps: int, warmup: int):

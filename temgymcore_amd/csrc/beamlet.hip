@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_kernel(const tg_beamlet_
                                                                float2* __restrict__ ws) {
   __shared__ TileLists sh;
   if (matrix_path(flags)) return;  // beamlet_sum_sep_kernel covers this launch
-  const int tile = blockIdx.x % ntiles, split = blockIdx.x / ntiles;
+  const int split = blockIdx.x % nsplit, tile = blockIdx.x / nsplit;  // slices of a tile adjacent
   const int32_t tx0 = (tile % ntx) * (LPR * RUN), ty0 = (tile / ntx) * (kThreads / LPR);
   const int64_t b_begin = n * split / nsplit, b_end = n * (split + 1) / nsplit;
   tile_sum_store<LPR, RUN>(coef, b_begin, b_end, tx0, ty0, (int32_t)H, (int32_t)W, sh, ws + (int64_t)split * H * W);
@@ -155,7 +155,7 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 constexpr int kSepK = 32;          // beamlets per LDS chunk (the product's K)
 constexpr int kSepSeg = 16;        // pixels per generated segment
 constexpr int kSepRow = kTileW + 1;  // LDS row pitch (float2): a bank offset per beamlet row
-constexpr int kSepBlock = 1024;    // beamlets per list build
+constexpr int kSepBlock = 2048;    // beamlets per list build
 constexpr double kLog2e = 1.4426950408889634;
 static_assert(kTileW == 64 && kTileH == 64 && kThreads == 256 && kThreads >= kFlagBlocks &&
                   kSepK * 2 * (kTileW / kSepSeg) == kThreads && kSepSeg == kSepK / 2 &&
@@ -246,7 +246,10 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_sep_kernel(const tg_beam
   __shared__ float2 fac[2][2][kSepK][kSepRow];  // [buffer][x: Ex_k(tx0 + n), y: Ey_k(ty0 + m)][k][pixel]
   if (!matrix_path(flags)) return;  // beamlet_sum_kernel covers this launch
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int tile = blockIdx.x % ntiles, split = blockIdx.x / ntiles;
+  // the slices of one tile are consecutive workgroups: the heavy tiles near
+  // the pattern's centre are spread through the dispatch order instead of
+  // all ending it
+  const int split = blockIdx.x % nsplit, tile = blockIdx.x / nsplit;
   const int32_t tx0 = (tile % ntx) * kTileW, ty0 = (tile / ntx) * kTileH;
   const int64_t b_begin = n * split / nsplit, b_end = n * (split + 1) / nsplit;
   const float ftx0 = (float)tx0, fty0 = (float)ty0, ftx1 = (float)(tx0 + kTileW - 1), fty1 = (float)(ty0 + kTileH - 1);
@@ -258,14 +261,20 @@ __global__ __launch_bounds__(kThreads) void beamlet_sum_sep_kernel(const tg_beam
 #pragma unroll
   for (int j = 0; j < 16; ++j) p1[j] = p2[j] = p3[j] = 0.0f;
   for (int64_t blk = b_begin; blk < b_end; blk += kSepBlock) {
-    // the beamlets of [blk, blk + kSepBlock) whose cutoff box meets the tile, in order
+    // the beamlets of [blk, blk + kSepBlock) whose cutoff ellipse meets the
+    // tile, in order.  A separable beamlet's ellipse is axis-aligned
+    // (alpha = 0): u^2 + gamma v^2 <= beta, so the tile's pixel centre nearest
+    // (cx, cy) decides, exactly -- tiles in the corners of the ellipse's
+    // bounding box are skipped.
     int cnt = 0;
     for (int64_t b0 = blk; b0 < b_end && b0 < blk + kSepBlock; b0 += kThreads) {
       const int64_t b = b0 + t;
       bool hit = false;
       if (b < b_end) {
         const tg_beamlet_coef& c = coef[b];
-        hit = c.cx + c.box_w >= ftx0 && c.cx - c.box_w <= ftx1 && c.cy + c.box_h >= fty0 && c.cy - c.box_h <= fty1;
+        const float du = fmaxf(fmaxf(ftx0 - c.cx, c.cx - ftx1), 0.0f);
+        const float dv = fmaxf(fmaxf(fty0 - c.cy, c.cy - fty1), 0.0f);
+        hit = fmaf(c.gamma * dv, dv, du * du) <= c.beta;
       }
       const uint64_t mask = __ballot(hit);
       if (lane == 0) wave_cnt[wave] = __popcll(mask);
