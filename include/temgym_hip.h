@@ -31,7 +31,7 @@ extern "C" {
 const char* tg_last_error(void);
 
 /* ABI version; bumped whenever a struct layout below changes. */
-int32_t tg_abi_version(void);  /* currently 6 */
+int32_t tg_abi_version(void);  /* currently 7 */
 
 /* ======================================================================== *
  * Ray branch: per-ray register programs

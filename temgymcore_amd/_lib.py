@@ -15,7 +15,7 @@ import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtemgym_hip.so")
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 TG_MAX_INSTR = 112
 TG_MAX_REG = 64

@@ -67,7 +67,7 @@ extern "C" {
 
 const char* tg_last_error(void) { return g_err; }
 
-int32_t tg_abi_version(void) { return 6; }
+int32_t tg_abi_version(void) { return 7; }
 
 int32_t tg_device_cu_count(void) { return tg::cu_count(); }
 
