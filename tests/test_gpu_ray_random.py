@@ -1,0 +1,100 @@
+"""Seeded random sweep of the ray-program kernel against the oracle.
+
+Each case draws a model of 1-7 elements from Lens, Deflector, Scanner,
+Descanner, Rotator, Biprism and Plane in random order at non-decreasing z
+(some at the previous element's z: the zero-distance propagation of the
+reference's step structure, run.py:75-82), a Detector with random pixel
+size, non-square shape (small ones on the LDS histogram, large ones on the
+global-atomic binning), rotation, centre and flip_y, and an f32 or f64
+bundle of ragged size with random starting z and pathlength.  Bar, as for
+the fixed models: positions, slopes and pathlength bit-exact, histogram
+exact (run_to_image with fused binning)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ray_oracle as O
+from temgymcore_amd.components import (
+    Biprism, Deflector, DescanError, Descanner, Detector, Lens, Plane, Rotator, Scanner,
+)
+from temgymcore_amd.ray import Ray
+from temgymcore_amd.run import run_to_image
+
+pytestmark = pytest.mark.gpu
+
+KINDS = ("lens", "deflector", "scanner", "descanner", "rotator", "biprism", "plane")
+
+
+def _element(rng, kind, z):
+    """(component, oracle tuple) of one element of the given kind at z."""
+    if kind == "lens":
+        f = float(rng.choice([-1.0, 1.0]) * rng.uniform(0.2, 2.0))
+        return Lens(z=z, focal_length=f), ("lens", z, f)
+    if kind == "deflector":
+        a, b = (float(v) for v in rng.normal(0.0, 0.01, 2))
+        return Deflector(z=z, def_x=a, def_y=b), ("deflector", z, a, b)
+    if kind in ("scanner", "descanner"):
+        px, py, tx, ty = (float(v) for v in rng.normal(0.0, 3e-3, 4))
+        if kind == "scanner":
+            return (Scanner(z=z, scan_pos_x=px, scan_pos_y=py, scan_tilt_x=tx, scan_tilt_y=ty),
+                    ("scanner", z, px, py, tx, ty))
+        err = tuple(float(v) for v in rng.normal(0.0, 1e-3, 12))
+        return (Descanner(z=z, scan_pos_x=px, scan_pos_y=py, scan_tilt_x=tx, scan_tilt_y=ty,
+                          descan_error=DescanError(*err)),
+                ("descanner", z, px, py, tx, ty, err))
+    if kind == "rotator":
+        ang = float(rng.uniform(-180.0, 180.0))
+        return Rotator(z=z, angle=ang), ("rotator", z, ang)
+    if kind == "biprism":
+        d = float(rng.normal(0.0, 1e-3))
+        return Biprism(z=z, def_x=d), ("biprism", z, d)
+    return Plane(z=z), ("plane", z)
+
+
+def _random_case(seed):
+    rng = np.random.default_rng(1000 + seed)
+    z0 = float(rng.uniform(-0.1, 0.1))
+    z = z0
+    model, omodel = [], []
+    for _ in range(int(rng.integers(1, 8))):
+        if rng.uniform() < 0.8:
+            z += float(rng.uniform(0.01, 0.3))
+        comp, ocomp = _element(rng, KINDS[int(rng.integers(len(KINDS)))], z)
+        model.append(comp)
+        omodel.append(ocomp)
+    z += float(rng.uniform(0.0, 0.5)) if rng.uniform() < 0.8 else 0.0
+    shape = (int(rng.integers(8, 640)), int(rng.integers(8, 640)))
+    pixel = (float(rng.uniform(1e-3, 1e-2)), float(rng.uniform(1e-3, 1e-2)))
+    centre = (float(rng.normal(0.0, 0.02)), float(rng.normal(0.0, 0.02)))
+    rotation = float(rng.uniform(-180.0, 180.0)) if rng.uniform() < 0.5 else 0.0
+    flip_y = bool(rng.uniform() < 0.3)
+    model.append(Detector(z=z, pixel_size=pixel, shape=shape, rotation=rotation, centre=centre, flip_y=flip_y))
+    omodel.append(("detector", z))
+    dtype = np.float32 if rng.uniform() < 0.6 else np.float64
+    n = int(rng.choice([1, 63, 4097, int(rng.integers(1, 250_000))]))
+    x, y = (rng.uniform(-0.1, 0.1, n).astype(dtype) for _ in range(2))
+    dx, dy = (rng.uniform(-0.05, 0.05, n).astype(dtype) for _ in range(2))
+    pl = rng.uniform(0.0, 1e-3, n).astype(dtype)
+    det = dict(centre=centre, pixel_size=pixel, shape=shape, flip_y=flip_y, rotation=rotation)
+    return model, omodel, (x, y, dx, dy, z0, pl), det
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_model_bit_exact(device, seed):
+    model, omodel, (x, y, dx, dy, z0, pl), det = _random_case(seed)
+    kinds = [type(c).__name__ for c in model]
+    print(f"[random ray model seed={seed} n={x.size} dtype={x.dtype} kinds={kinds} shape={det['shape']}]",
+          flush=True)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
+    ray = Ray(x=dev(x), y=dev(y), dx=dev(dx), dy=dev(dy), z=z0, pathlength=dev(pl))
+    out, img = run_to_image(ray, model)
+    torch.cuda.synchronize()
+    ref = O.run_to_end(O.make_ray(x, y, dx, dy, z0, pl), omodel)
+    for k in ("x", "y", "dx", "dy", "pathlength"):
+        got = getattr(out, k).cpu().numpy()
+        assert got.dtype == x.dtype, k
+        np.testing.assert_array_equal(got, ref[k], err_msg=f"{k} (seed {seed})")
+    Tinv = O.metres_to_pixels_matrix(det["centre"], det["pixel_size"], det["shape"], det["flip_y"], det["rotation"])
+    py, px = O.metres_to_pixels(ref["x"], ref["y"], Tinv)
+    expect = O.into_image(py, px, det["shape"])
+    np.testing.assert_array_equal(img.cpu().numpy().astype(np.int64), expect, err_msg=f"histogram (seed {seed})")
