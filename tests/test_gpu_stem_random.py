@@ -93,3 +93,31 @@ def test_random_geometry_fit_gradient(device, seed):
         bar = 2e-4 * np.abs(ref_g[blk]).max() + floor[blk]
         err = np.abs(g[blk] - ref_g[blk])
         assert np.all(err <= bar), (blk, err / bar)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_geometry_backproject(device, seed):
+    """Back-projection of constant detector images per scan position (a ray on
+    a pixel edge cannot matter) onto random sample grids, against the
+    oracle's sums: weights within 1e-6 and images within 2e-6 of the largest
+    weight, as for the fixed geometries.  40 um detector pixels: every ray
+    lands."""
+    import dataclasses
+
+    rng = np.random.default_rng(6000 + seed)
+    sides = (2, 16, 38, 64, 100, 74, 256, 30)
+    model = dataclasses.replace(_random_model(rng, max_scan=9, n_side=sides[seed % len(sides)], max_cam=3),
+                                det_pixel=4e-5)
+    cam = int(rng.integers(model.n_cam))
+    print(f"[random stem backproject seed={seed} scan={model.scan_shape} n_side={model.n_side} cam={cam}/"
+          f"{model.n_cam} sample={model.sample_shape}]", flush=True)
+    c = rng.uniform(0.5, 2.0, model.n_scan)
+    D = np.broadcast_to(c[:, None, None], (model.n_scan,) + tuple(model.det_shape)).astype(np.float32)
+    img, w = stem.backproject(model, torch.from_numpy(np.ascontiguousarray(D)).to(device), cam=cam, weights=True)
+    geom = dict(_geom(model), sample_shape=tuple(model.sample_shape))
+    det_Tinv = R.metres_to_pixels_matrix((0.0, 0.0), (model.det_pixel,) * 2, model.det_shape)
+    img_Tinv = R.metres_to_pixels_matrix((0.0, 0.0), (model.sample_pixel,) * 2, model.sample_shape)
+    ref_img, ref_w = SO.backproject(geom, model.params(), D.astype(np.float64), det_Tinv, img_Tinv, cam)
+    scale = max(ref_w.max(), 1e-300)
+    np.testing.assert_allclose(w.cpu().numpy(), ref_w, rtol=0, atol=1e-6 * scale)
+    np.testing.assert_allclose(img.cpu().numpy(), ref_img, rtol=0, atol=1e-6 * 2.0 * scale)
