@@ -249,6 +249,14 @@ def fibonacci_spiral(nb_samples, radius, alpha=2):
     return rr * np.cos(i * ga), rr * np.sin(i * ga)
 
 
+def thick_lens(r, z_po, z_pi, f):
+    """ThickLens (reference components.py:410-451): the thin-lens update at its
+    object plane, then the ray's z moves to the image plane."""
+    r = lens(r, f)
+    r["z"] = r["z"] - (z_po - z_pi)
+    return r
+
+
 def apply(r, comp):
     kind, *p = comp
     if kind in ("plane", "detector"):
@@ -267,6 +275,8 @@ def apply(r, comp):
         return aberrated_lens(r, p[1], p[2])
     if kind == "biprism":
         return biprism(r, p[1])
+    if kind == "thicklens":  # ("thicklens", z_po, z_pi, f)
+        return thick_lens(r, p[0], p[1], p[2])
     raise ValueError(kind)
 
 

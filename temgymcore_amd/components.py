@@ -44,8 +44,9 @@ class Component(HasParamsMixin):
     """Base component: pure ``__call__(ray) -> Ray`` at axial position ``z``.
 
     Subclasses write the per-ray update with ordinary arithmetic on the ray's
-    fields; it is compiled for the GPU automatically.  Components never move
-    ``ray.z`` -- propagators do (reference components.py:12-24).
+    fields; it is compiled for the GPU automatically.  Components do not move
+    ``ray.z`` -- propagators do (reference components.py:12-24) -- except
+    ThickLens, which hands the ray on from its object to its image plane.
     """
 
     def __init_subclass__(cls, **kw):
@@ -123,6 +124,32 @@ class Lens(Component):
         pathlength = ray.pathlength - (x**2 + y**2) / (2 * f)
         one = ray._one * 1.0
         return Ray(x=x, y=y, dx=new_dx, dy=new_dy, _one=one, pathlength=pathlength, z=ray.z)
+
+
+@dataclasses.dataclass(frozen=True)
+class ThickLens(Component):
+    """Thick lens between an object plane z_po and an image plane z_pi: the
+    thin-lens update of the slopes and pathlength at z_po, after which the
+    ray continues from z_pi (its z moves by z_pi - z_po).  Its position
+    ``z`` is z_po (reference components.py:410-451)."""
+
+    z_po: float
+    z_pi: float
+    focal_length: float
+
+    @property
+    def z(self):
+        return self.z_po
+
+    def __call__(self, ray: Ray):
+        f = self.focal_length
+        x, y, dx, dy = ray.x, ray.y, ray.dx, ray.dy
+        new_dx = -x / f + dx
+        new_dy = -y / f + dy
+        pathlength = ray.pathlength - (x**2 + y**2) / (2 * f)
+        new_z = ray.z - (self.z_po - self.z_pi)
+        one = ray._one * 1.0
+        return Ray(x=x, y=y, dx=new_dx, dy=new_dy, _one=one, pathlength=pathlength, z=new_z)
 
 
 @dataclasses.dataclass(frozen=True)

@@ -36,12 +36,25 @@ def passthrough_transform(component):
     return inner
 
 
-def jacobian_transform(component):
-    """Per-step (ray_out, jacobian) for scalar rays, via forward-mode duals."""
+def jacobian_and_value(fn, argnums: int = 0):
+    """``f(ray, *args) -> (d fn / d ray, fn(ray, *args))`` for a scalar ray, by
+    forward-mode duals (reference run.py:17-23, which uses jax.jacobian).
+    Only the ray argument (argnums=0) is differentiated."""
+    if argnums != 0:
+        raise NotImplementedError("jacobian_and_value differentiates the ray argument (argnums=0) only")
     from .transfer import jacobian
 
+    def inner(ray, *args, **kwargs):
+        return jacobian(lambda r: fn(r, *args, **kwargs), has_aux_value=True)(ray)
+
+    return inner
+
+
+def jacobian_transform(component):
+    """Per-step (ray_out, jacobian) for scalar rays, via forward-mode duals."""
+
     def inner(ray):
-        jac, out = jacobian(component, has_aux_value=True)(ray)
+        jac, out = jacobian_and_value(component)(ray)
         return out, jac
 
     return inner
@@ -160,6 +173,18 @@ def plan_run_to_end(ray: Ray, components: Sequence, propagator: BasePropagator =
                     composed: bool = False) -> "_program.Plan":
     """Compile ``run_to_end`` for a vectorised ray once; ``plan.launch()`` runs it."""
     return _program.plan_fn(_model_fn(ray, components, propagator, composed), ray)
+
+
+def calculate_derivatives(ray: Ray, model: Sequence, order: int):
+    """Derivatives of run_to_end with respect to a scalar ray, orders 1 ..
+    `order` (reference run.py:119-147, successive jax.jacfwd).  The dual
+    numbers here are first order: order 1 gives [jacobian]; higher orders
+    raise NotImplementedError."""
+    if order > 1:
+        raise NotImplementedError("calculate_derivatives: only first derivatives (order 1) are available")
+    from .transfer import jacobian
+
+    return [jacobian(lambda r: run_to_end(r, model))(ray)] if order == 1 else []
 
 
 def solve_model(ray: Ray, model: Sequence, propagator: BasePropagator = FreeSpaceParaxial()):

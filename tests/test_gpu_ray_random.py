@@ -1,7 +1,7 @@
 """Seeded random sweep of the ray-program kernel against the oracle.
 
-Each case draws a model of 1-7 elements from Lens, Deflector, Scanner,
-Descanner, Rotator, Biprism and Plane in random order at non-decreasing z
+Each case draws a model of 1-7 elements from Lens, ThickLens, Deflector,
+Scanner, Descanner, Rotator, Biprism and Plane in random order at non-decreasing z
 (some at the previous element's z: the zero-distance propagation of the
 reference's step structure, run.py:75-82), a Detector with random pixel
 size, non-square shape (small ones on the LDS histogram, large ones on the
@@ -15,14 +15,14 @@ import torch
 
 from oracle import ray_oracle as O
 from temgymcore_amd.components import (
-    Biprism, Deflector, DescanError, Descanner, Detector, Lens, Plane, Rotator, Scanner,
+    Biprism, Deflector, DescanError, Descanner, Detector, Lens, Plane, Rotator, Scanner, ThickLens,
 )
 from temgymcore_amd.ray import Ray
 from temgymcore_amd.run import run_to_image
 
 pytestmark = pytest.mark.gpu
 
-KINDS = ("lens", "deflector", "scanner", "descanner", "rotator", "biprism", "plane")
+KINDS = ("lens", "deflector", "scanner", "descanner", "rotator", "biprism", "plane", "thicklens")
 
 
 def _element(rng, kind, z):
@@ -48,6 +48,10 @@ def _element(rng, kind, z):
     if kind == "biprism":
         d = float(rng.normal(0.0, 1e-3))
         return Biprism(z=z, def_x=d), ("biprism", z, d)
+    if kind == "thicklens":
+        f = float(rng.choice([-1.0, 1.0]) * rng.uniform(0.2, 2.0))
+        z_pi = z + float(rng.uniform(0.0, 0.1))
+        return ThickLens(z_po=z, z_pi=z_pi, focal_length=f), ("thicklens", z, z_pi, f)
     return Plane(z=z), ("plane", z)
 
 
@@ -62,6 +66,8 @@ def _random_case(seed):
         comp, ocomp = _element(rng, KINDS[int(rng.integers(len(KINDS)))], z)
         model.append(comp)
         omodel.append(ocomp)
+        if ocomp[0] == "thicklens":
+            z = ocomp[2]  # the ray continues from the image plane
     z += float(rng.uniform(0.0, 0.5)) if rng.uniform() < 0.8 else 0.0
     shape = (int(rng.integers(8, 640)), int(rng.integers(8, 640)))
     pixel = (float(rng.uniform(1e-3, 1e-2)), float(rng.uniform(1e-3, 1e-2)))

@@ -397,3 +397,71 @@ def test_traced_jacobian_with_many_outputs_is_split_by_outputs(monkeypatch):
     for b in range(n):
         r = Ray(float(cols["x"][b]), float(cols["y"][b]), float(cols["dx"][b]), float(cols["dy"][b]), 0.0, 0.0)
         np.testing.assert_allclose(J[b], np.asarray(ray_matrix(model, r)), rtol=1e-12, atol=1e-15)
+
+
+def test_thick_lens_matches_oracle():
+    """ThickLens (reference components.py:410-451) on a scalar ray, traced for
+    f32 bundles (interpreted here) and as a 5x5 matrix, against the oracle."""
+    from temgymcore_amd.components import ThickLens
+
+    model = [ThickLens(z_po=0.5, z_pi=0.6, focal_length=1.0), Detector(z=1.0, pixel_size=(0.01, 0.01), shape=(64, 64))]
+    omodel = [("thicklens", 0.5, 0.6, 1.0), ("detector", 1.0)]
+    out = run_to_end(Ray(x=0.1, y=0.2, dx=0.3, dy=0.4, z=0.0, pathlength=0.0), model)
+    ref = O.run_to_end(O.make_ray(0.1, 0.2, 0.3, 0.4), omodel)
+    for k in ("x", "y", "dx", "dy", "z", "pathlength"):
+        assert getattr(out, k) == ref[k], k
+    x, y, dx, dy = O.plain_workload(4097, seed=11)
+    ray = Ray(x=torch.from_numpy(x), y=torch.from_numpy(y), dx=torch.from_numpy(dx), dy=torch.from_numpy(dy),
+              z=0.0, pathlength=0.0)
+    res, _ = run_host(ray, model)
+    ref = O.run_to_end(O.make_ray(x, y, dx, dy), omodel)
+    for k in ("x", "y", "dx", "dy", "pathlength"):
+        np.testing.assert_array_equal(res[k], ref[k], err_msg=k)
+    assert res["z"] == ref["z"]
+    T = ray_matrix(model, Ray(0.0, 0.0, 0.0, 0.0, 0.0, 0.0))
+    np.testing.assert_allclose(T[:4, :4], [[0.6, 0, 0.7, 0], [0, 0.6, 0, 0.7], [-1, 0, 0.5, 0], [0, -1, 0, 0.5]],
+                               atol=1e-15)
+
+
+def test_dir_cosine_propagator_closed_form_and_device_rays():
+    """FreeSpaceDirCosine on a scalar ray against the direction-cosine closed
+    form (reference tests/test_rays.py:59-79, seeded distance); traced for a
+    per-ray bundle it has no square-root instruction and says so."""
+    import math
+
+    from temgymcore_amd.propagator import FreeSpaceDirCosine
+
+    d = float(np.random.default_rng(3).uniform(-10.0, 10.0))
+    x0, y0, dx0, dy0, z0, pl0 = 1.0, -1.0, 2.0, 3.0, 0.5, 10.0
+    out = FreeSpaceDirCosine()(Ray(x=x0, y=y0, dx=dx0, dy=dy0, z=z0, pathlength=pl0), d)
+    n = math.sqrt(1.0 + dx0**2 + dy0**2)
+    assert out.x == pytest.approx(x0 + (dx0 / n) / n * d, abs=1e-12)
+    assert out.y == pytest.approx(y0 + (dy0 / n) / n * d, abs=1e-12)
+    assert (out.dx, out.dy) == (dx0, dy0)
+    assert out.z == pytest.approx(z0 + d, abs=1e-12)
+    assert out.pathlength == pytest.approx(pl0 + d * n, abs=1e-12)
+    x = torch.linspace(-0.1, 0.1, 8)
+    tr = _program.Trace(torch.device("cpu"))
+    ray = tr.symbolize(Ray(x=x, y=x.clone(), dx=x * 0.1, dy=x * 0.2, z=0.0, pathlength=0.0))
+    with pytest.raises(NotImplementedError, match="square root"):
+        _trace_model(ray, [Detector(z=1.0, pixel_size=(0.01, 0.01), shape=(8, 8))], FreeSpaceDirCosine())
+
+
+def test_jacobian_and_value_and_first_derivatives():
+    """jacobian_and_value / calculate_derivatives (reference run.py:17-23,
+    :119-147) on the README model: the README's ABCD and value."""
+    from temgymcore_amd.run import calculate_derivatives, jacobian_and_value
+    from temgymcore_amd.transfer import custom_jacobian_matrix
+
+    model = [Lens(z=0.5, focal_length=1.0), Detector(z=1.0, pixel_size=(0.01, 0.01), shape=(128, 128))]
+    ray = Ray(x=0.1, y=0.2, dx=0.3, dy=0.4, z=0.0, pathlength=0.0)
+    jac, out = jacobian_and_value(run_to_end)(ray, model)
+    readme = np.array([[.5, 0, .75, 0, 0], [0, .5, 0, .75, 0], [-1, 0, .5, 0, 0], [0, -1, 0, .5, 0], [0, 0, 0, 0, 1]])
+    np.testing.assert_allclose(custom_jacobian_matrix(jac), readme, atol=1e-15)
+    assert out.x == pytest.approx(0.275, abs=1e-15) and out.pathlength == pytest.approx(0.88875, abs=1e-15)
+    (d1,) = calculate_derivatives(ray, model, 1)
+    np.testing.assert_allclose(custom_jacobian_matrix(d1), readme, atol=1e-15)
+    with pytest.raises(NotImplementedError):
+        calculate_derivatives(ray, model, 2)
+    with pytest.raises(NotImplementedError):
+        jacobian_and_value(run_to_end, argnums=1)

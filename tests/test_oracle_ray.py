@@ -439,3 +439,24 @@ def test_biprism_planted_mistakes_fail(plant, monkeypatch):
         except AssertionError:
             caught += 1
     assert caught >= 1, "a planted mistake in the biprism passed every pin"
+
+
+def test_thick_lens_closed_form():
+    """ThickLens(z_po=0.5, z_pi=0.6, f=1) then a detector at 1.0, on the README
+    ray: thin lens at 0.5 (x = 0.25, dx = 0.3 - 0.25 = 0.05), the ray moves to
+    0.6, then 0.4 m of drift: x = 0.27, y = 0.4 + 0 * 0.4 = 0.4; pathlength
+    0.5 - (0.25^2 + 0.4^2) / 2 + 0.4 = 0.78875.  Per axis the ABCD is
+    drift(0.4) lens(1) drift(0.5) = [[0.6, 0.7], [-1, 0.5]].  (Moving z the
+    wrong way would end at x = 0.28.)"""
+    model = [("thicklens", 0.5, 0.6, 1.0), ("detector", 1.0)]
+    r = O.run_to_end(O.make_ray(0.1, 0.2, 0.3, 0.4), model)
+    assert r["x"] == pytest.approx(0.27, abs=1e-15)
+    assert r["y"] == pytest.approx(0.4, abs=1e-15)
+    assert r["dx"] == pytest.approx(0.05, abs=1e-15)
+    assert r["dy"] == pytest.approx(0.0, abs=1e-15)
+    assert r["z"] == pytest.approx(1.0, abs=1e-15)
+    assert r["pathlength"] == pytest.approx(0.78875, abs=1e-15)
+    J = _fd_matrix(model, O.make_ray(0.0, 0.0, 0.0, 0.0))
+    expect = np.array([[0.6, 0, 0.7, 0, 0], [0, 0.6, 0, 0.7, 0], [-1, 0, 0.5, 0, 0], [0, -1, 0, 0.5, 0],
+                       [0, 0, 0, 0, 1.0]])
+    np.testing.assert_allclose(J, expect, atol=1e-8)
