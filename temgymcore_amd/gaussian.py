@@ -94,6 +94,49 @@ def q_inv(z, w0, wl):
     return -1.0 / R(z, z_r) + 1j * wl / (np.pi * w_z(w0, z, z_r) ** 2)
 
 
+def decompose_Q_inv(Q_inv, wavelength, eps=1e-12):
+    """Beam parameters (waist_x, waist_y, r_x, r_y, theta) of 2x2 complex
+    Q_inv matrices [..., 2, 2] (reference gaussian.py:35-89).
+
+    The principal axes are the eigenvectors of the symmetrised imaginary part
+    (a proper rotation), the larger waist first; on each axis
+    Im(1/q) = wavelength / (pi w^2) gives the waist (inf where |Im| <= eps)
+    and the radius is 1 / Re(1/q) (inf where |Re| <= eps).  As in the
+    reference, that radius is the negative of GaussianRay's radii_of_curv
+    convention (1/q = -1/R + i ...): rebuilding from it keeps |field| and
+    mirrors the curvature.  theta is the first axis' angle, taken in
+    (-pi/2, pi/2] (an eigenvector's sign is arbitrary; the ellipse is the
+    same).  numpy arrays or tensors in, numpy arrays out."""
+    Q = np.asarray(_host(Q_inv), dtype=np.complex128)
+    S = Q.imag
+    S = 0.5 * (S + np.swapaxes(S, -1, -2))
+    _, V = np.linalg.eigh(S)
+    V = V.copy()
+
+    def proper(V):
+        flip = np.where(np.linalg.det(V) < 0, -1.0, 1.0)
+        V[..., :, 1] *= flip[..., None]
+        return V
+
+    V = proper(V)
+    Qd = np.swapaxes(V, -1, -2) @ Q @ V
+    qd = np.stack([Qd[..., 0, 0], Qd[..., 1, 1]], axis=-1)
+
+    def waists_of(im):
+        with np.errstate(divide="ignore"):
+            return np.sqrt(np.where(np.abs(im) > eps, np.abs(wavelength / (np.pi * im)), np.inf))
+
+    swap = waists_of(qd.imag)[..., 0] < waists_of(qd.imag)[..., 1]
+    qd = np.where(swap[..., None], qd[..., ::-1], qd)
+    V = proper(np.where(swap[..., None, None], V[..., :, ::-1], V))
+    waists = waists_of(qd.imag)
+    with np.errstate(divide="ignore"):
+        radii = np.where(np.abs(qd.real) > eps, 1.0 / qd.real, np.inf)
+    theta = np.arctan2(V[..., 1, 0], V[..., 0, 0])
+    theta = np.where(theta > np.pi / 2, theta - np.pi, np.where(theta <= -np.pi / 2, theta + np.pi, theta))
+    return waists[..., 0], waists[..., 1], radii[..., 0], radii[..., 1], theta
+
+
 # ---- GaussianRay -------------------------------------------------------------------
 
 @dataclasses.dataclass(frozen=True, kw_only=True)

@@ -165,3 +165,45 @@ def test_Qinv_ABCD_identity_and_free_space():
     f = 0.7
     Q3 = G.Qinv_ABCD(Q1, I, Z, -I / f, I)
     np.testing.assert_allclose(Q3, Q1 - I / f, rtol=1e-12)
+
+
+def test_decompose_q_inv_round_trip():
+    """decompose_Q_inv (reference gaussian.py:35-89) of GaussianRay.Q_inv
+    (reference tests/test_gaussians.py:893-950's beams, batched): the larger
+    waist first with theta its axis (mod pi), the smaller second; radii are
+    1 / Re(1/q), i.e. minus radii_of_curv (the reference's convention), and
+    rebuilding with them reproduces |Q_inv| entries' imaginary parts and the
+    real parts negated."""
+    import torch
+
+    from temgymcore_amd.gaussian import GaussianRay, decompose_Q_inv
+
+    wl = 1.3e-6
+    params = [(1e-5, 3e-5, np.inf, np.inf, np.pi / 6), (1.2e-4, 5e-5, 1e-4, 0.1, -np.pi / 4),
+              (1e-4, 3e-4, 0.01, np.inf, -np.pi / 3)]
+    w = np.array([[a, b] for a, b, *_ in params])
+    r = np.array([[c, d] for _, _, c, d, _ in params])
+    th = np.array([p[4] for p in params])
+    n = len(params)
+    gr = GaussianRay(x=torch.zeros(n), y=torch.zeros(n), dx=torch.zeros(n), dy=torch.zeros(n), z=0.0,
+                     pathlength=0.0, _one=1.0, amplitude=torch.ones(n), waist_xy=torch.from_numpy(w),
+                     radii_of_curv=torch.from_numpy(r), wavelength=wl, theta=torch.from_numpy(th))
+    Q = gr.Q_inv
+    w1, w2, r1, r2, t = decompose_Q_inv(Q, wl)
+    for i, (a, b, c, d, ang) in enumerate(params):
+        big, small = (a, b) if a >= b else (b, a)
+        rbig, rsmall = (c, d) if a >= b else (d, c)
+        axis = ang if a >= b else ang + np.pi / 2
+        assert w1[i] == pytest.approx(big, rel=1e-9) and w2[i] == pytest.approx(small, rel=1e-9)
+        assert (r1[i] == np.inf if np.isinf(rbig) else r1[i] == pytest.approx(-rbig, rel=1e-9))
+        assert (r2[i] == np.inf if np.isinf(rsmall) else r2[i] == pytest.approx(-rsmall, rel=1e-9))
+        diff = np.arctan2(np.sin(t[i] - axis), np.cos(t[i] - axis))
+        assert min(abs(diff), abs(abs(diff) - np.pi)) < 1e-9
+        assert -np.pi / 2 < t[i] <= np.pi / 2
+    back = GaussianRay(x=torch.zeros(n), y=torch.zeros(n), dx=torch.zeros(n), dy=torch.zeros(n), z=0.0,
+                       pathlength=0.0, _one=1.0, amplitude=torch.ones(n),
+                       waist_xy=torch.from_numpy(np.stack([w1, w2], -1)),
+                       radii_of_curv=torch.from_numpy(np.stack([r1, r2], -1)), wavelength=wl,
+                       theta=torch.from_numpy(t)).Q_inv
+    np.testing.assert_allclose(back.imag, Q.imag, rtol=1e-9, atol=1e-9 * np.abs(Q.imag).max())
+    np.testing.assert_allclose(back.real, -Q.real, rtol=1e-9, atol=1e-9 * np.abs(Q.real).max())
