@@ -137,6 +137,68 @@ def decompose_Q_inv(Q_inv, wavelength, eps=1e-12):
     return waists[..., 0], waists[..., 1], radii[..., 0], radii[..., 1], theta
 
 
+# ---- small algebra helpers (reference gaussian.py:180-222, :340-369) ------------------
+
+def _einsum(spec, *ops):
+    if any(isinstance(o, torch.Tensor) for o in ops):
+        return torch.einsum(spec, *[torch.as_tensor(o) for o in ops])
+    return np.einsum(spec, *ops)
+
+
+def matrix_vector_mul(M, v):
+    """M @ v for one 2x2 M and 2-vector v (reference gaussian.py:180-187)."""
+    return _einsum("ij,j->i", M, v)
+
+
+def matrix_matrix_mul(M1, M2):
+    """M1 @ M2 for 2x2 matrices (reference gaussian.py:190-197)."""
+    return _einsum("ij,jk->ik", M1, M2)
+
+
+def matrix_quadratic_mul(v, M):
+    """v^T M v (reference gaussian.py:200-207)."""
+    return _einsum("i,ij,j->", v, M, v)
+
+
+def matrix_linear_mul(v, M, w):
+    """v^T M w_n for the rows w_n of w [np, 2]: shape [np] (reference
+    gaussian.py:210-218)."""
+    return _einsum("i,ij,nj->n", v, M, w)
+
+
+def matrix_matrix_matrix_mul(M1, M2, M3):
+    """M1_n M2_n M3_n^T over a batch [n, 2, 2] (reference gaussian.py:221-222)."""
+    return _einsum("nij,njk,npk->nip", M1, M2, M3)
+
+
+def _leading(xs):
+    if isinstance(xs, dict):
+        return len(next(iter(xs.values())))
+    if isinstance(xs, (tuple, list)):
+        return len(xs[0])
+    return len(xs)
+
+
+def _take(xs, i):
+    if isinstance(xs, dict):
+        return {k: v[i] for k, v in xs.items()}
+    if isinstance(xs, (tuple, list)):
+        return type(xs)(v[i] for v in xs)
+    return xs[i]
+
+
+def map_reduce(f, reducer, init, xs, *, batch_size: int | None = None):
+    """The fold reducer(... reducer(reducer(init, f(x_0)), f(x_1)) ..., f(x_n-1))
+    over the leading axis of xs (an array/tensor, or a tuple, list or dict of
+    them with one leading length) (reference gaussian.py:340-369, lax.scan;
+    its batch_size only groups f's evaluations, the fold order is the same,
+    so it is accepted and ignored here)."""
+    acc = init
+    for i in range(_leading(xs)):
+        acc = reducer(acc, f(_take(xs, i)))
+    return acc
+
+
 # ---- GaussianRay -------------------------------------------------------------------
 
 @dataclasses.dataclass(frozen=True, kw_only=True)

@@ -207,3 +207,31 @@ def test_decompose_q_inv_round_trip():
                        theta=torch.from_numpy(t)).Q_inv
     np.testing.assert_allclose(back.imag, Q.imag, rtol=1e-9, atol=1e-9 * np.abs(Q.imag).max())
     np.testing.assert_allclose(back.real, -Q.real, rtol=1e-9, atol=1e-9 * np.abs(Q.real).max())
+
+
+def test_gaussian_algebra_helpers_and_map_reduce():
+    """The reference's 2x2 helpers (gaussian.py:180-222) against explicit
+    products, on numpy and torch inputs, and map_reduce (:340-369) as a
+    left fold over the leading axis of arrays, tuples and dicts."""
+    import torch
+
+    from temgymcore_amd import gaussian as G
+
+    rng = np.random.default_rng(4)
+    M1, M2 = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)), rng.normal(size=(2, 2))
+    v, w = rng.normal(size=2), rng.normal(size=(5, 2))
+    np.testing.assert_allclose(G.matrix_vector_mul(M1, v), M1 @ v)
+    np.testing.assert_allclose(G.matrix_matrix_mul(M1, M2), M1 @ M2)
+    assert G.matrix_quadratic_mul(v, M1) == pytest.approx(v @ M1 @ v)
+    np.testing.assert_allclose(G.matrix_linear_mul(v, M1, w), np.array([v @ M1 @ wn for wn in w]))
+    B = rng.normal(size=(3, 2, 2))
+    np.testing.assert_allclose(G.matrix_matrix_matrix_mul(B, B, B), B @ B @ np.swapaxes(B, -1, -2))
+    t = G.matrix_vector_mul(torch.from_numpy(M2), torch.from_numpy(v))
+    assert isinstance(t, torch.Tensor) and np.allclose(t.numpy(), M2 @ v)
+    xs = np.arange(1.0, 6.0)
+    assert G.map_reduce(lambda x: x * x, lambda a, b: a + b, 0.0, xs) == pytest.approx(55.0)
+    assert G.map_reduce(lambda x: x, lambda a, b: a * 10 + b, 0.0, xs) == pytest.approx(12345.0)  # order
+    pairs = (np.array([1.0, 2.0]), np.array([3.0, 4.0]))
+    assert G.map_reduce(lambda p: p[0] * p[1], lambda a, b: a + b, 0.0, pairs) == pytest.approx(11.0)
+    d = {"a": np.array([1.0, 2.0]), "b": np.array([5.0, 7.0])}
+    assert G.map_reduce(lambda p: p["b"] - p["a"], lambda a, b: a + b, 0.0, d, batch_size=1) == pytest.approx(9.0)
