@@ -440,6 +440,16 @@ def test_dir_cosine_propagator_closed_form_and_device_rays():
     assert (out.dx, out.dy) == (dx0, dy0)
     assert out.z == pytest.approx(z0 + d, abs=1e-12)
     assert out.pathlength == pytest.approx(pl0 + d * n, abs=1e-12)
+    # the same numbers written out: 1 + dx^2 + dy^2 = 14, so x moves by 2 d / 14
+    # (the reference divides by N twice) and the path by d sqrt(14)
+    assert (out.x, out.y) == (pytest.approx(1.0 + d / 7.0, abs=1e-12), pytest.approx(-1.0 + 3.0 * d / 14.0, abs=1e-12))
+    assert out.pathlength == pytest.approx(10.0 + d * 14.0**0.5, abs=1e-12)
+    # small slopes: the paraxial step to third order in the slope
+    s = 1e-4
+    par = FreeSpaceParaxial()(Ray(x=x0, y=y0, dx=s, dy=-s, z=z0, pathlength=pl0), d)
+    dc = FreeSpaceDirCosine()(Ray(x=x0, y=y0, dx=s, dy=-s, z=z0, pathlength=pl0), d)
+    assert abs(dc.x - par.x) <= 3 * s**3 * abs(d) and abs(dc.y - par.y) <= 3 * s**3 * abs(d)
+    assert abs(dc.pathlength - par.pathlength) <= 2 * s**2 * abs(d)
     x = torch.linspace(-0.1, 0.1, 8)
     tr = _program.Trace(torch.device("cpu"))
     ray = tr.symbolize(Ray(x=x, y=x.clone(), dx=x * 0.1, dy=x * 0.2, z=0.0, pathlength=0.0))
