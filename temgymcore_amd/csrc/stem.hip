@@ -283,6 +283,8 @@ __device__ __forceinline__ int bp_slot(const tg_stem_geometry& G, int oy, int ox
   return ty_ >= 0 && ty_ < kBpTile && tx_ >= 0 && tx_ < kBpTile ? ty_ * kBpTile + tx_ : kBpGlobal;
 }
 
+// WEIGHT: also accumulate weight[q] (the sum of w_q), else weight is unused.
+template <bool WEIGHT>
 __global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_stem_geometry G,
                                                                     const double* __restrict__ P,
                                                                     const float* __restrict__ data, int cam,
@@ -290,14 +292,14 @@ __global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_ste
                                                                     double* __restrict__ weight) {
   __shared__ float th[kMaxSide];
   __shared__ double t_img[kBpTile * kBpTile];
-  __shared__ double t_w[kBpTile * kBpTile];
+  __shared__ double t_w[WEIGHT ? kBpTile * kBpTile : 1];
   __shared__ float sh_a, sh_bx, sh_by;
   const int64_t s = blockIdx.y;
   const int n = G.n_side;
   for (int k = threadIdx.x; k < n; k += kThreads) th[k] = slope_of(G, k);
   for (int k = threadIdx.x; k < kBpTile * kBpTile; k += kThreads) {
     t_img[k] = 0.0;
-    t_w[k] = 0.0;
+    if (WEIGHT) t_w[k] = 0.0;
   }
   const ScanPos p = scan_position(G, P, s);
   if (threadIdx.x == 0) {
@@ -320,112 +322,174 @@ __global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_ste
   const int iy0 = (int)(blockIdx.x / ntx) * kBpRayTile + (int)(threadIdx.x / 16) * kBpBlock;
   const int ix0 = (int)(blockIdx.x % ntx) * kBpRayTile + (int)(threadIdx.x % 16) * kBpBlock;
   if (iy0 < n && ix0 < n) {
+    // The sample position fma(ky_y, ty, fma(ky_x, tx, fy0)) (likewise fx) and
+    // the detector coordinate ((m0 yd) + (m1 xd)) + m2 (likewise fx) take
+    // their inner terms from the ray's column (tx, xd) or row (ty, yd) alone:
+    // computed once per row and column of the block, with the same roundings
+    // as the per-ray expressions of stem_forward_kernel.  Rows and columns
+    // past n repeat the last one (their rays are masked out below).
+    double cy[kBpBlock], cx[kBpBlock], tyd[kBpBlock];
+    double ry0[kBpBlock], ry3[kBpBlock], cx1[kBpBlock], cx4[kBpBlock];
+#pragma unroll
+    for (int k = 0; k < kBpBlock; ++k) {
+      const float tx = th[min(ix0 + k, n - 1)], ty = th[min(iy0 + k, n - 1)];
+      cy[k] = fma(smap.ky_x, (double)tx, smap.fy0);
+      cx[k] = fma(smap.kx_x, (double)tx, smap.fx0);
+      tyd[k] = (double)ty;
+      const float xd = fmaf(tx, sh_a, sh_bx), yd = fmaf(ty, sh_a, sh_by);
+      ry0[k] = __dmul_rn(G.det_m[0], (double)yd);
+      ry3[k] = __dmul_rn(G.det_m[3], (double)yd);
+      cx1[k] = __dmul_rn(G.det_m[1], (double)xd);
+      cx4[k] = __dmul_rn(G.det_m[4], (double)xd);
+    }
     // window anchor: floor of the smallest sample coordinates over the
     // block's corners (the position is affine in the slopes)
-    const int iy1 = min(iy0 + kBpBlock - 1, n - 1), ix1 = min(ix0 + kBpBlock - 1, n - 1);
-    double cy0, cx0, cy1, cx1, cy2, cx2, cy3, cx3;
-    sample_pixel(smap, th[ix0], th[iy0], cy0, cx0);
-    sample_pixel(smap, th[ix1], th[iy0], cy1, cx1);
-    sample_pixel(smap, th[ix0], th[iy1], cy2, cx2);
-    sample_pixel(smap, th[ix1], th[iy1], cy3, cx3);
-    const double wy0 = floor(fmin(fmin(cy0, cy1), fmin(cy2, cy3)));
-    const double wx0 = floor(fmin(fmin(cx0, cx1), fmin(cx2, cx3)));
+    constexpr int L = kBpBlock - 1;
+    const double wy0 = floor(fmin(fmin(fma(smap.ky_y, tyd[0], cy[0]), fma(smap.ky_y, tyd[0], cy[L])),
+                                  fmin(fma(smap.ky_y, tyd[L], cy[0]), fma(smap.ky_y, tyd[L], cy[L]))));
+    const double wx0 = floor(fmin(fmin(fma(smap.kx_y, tyd[0], cx[0]), fma(smap.kx_y, tyd[0], cx[L])),
+                                  fmin(fma(smap.kx_y, tyd[L], cx[0]), fma(smap.kx_y, tyd[L], cx[L]))));
     // anchors far outside the image: every ray deposits directly (and skips)
     const bool anchored = wy0 > -4.0 && wy0 < (double)G.img_h + 4.0 && wx0 > -4.0 && wx0 < (double)G.img_w + 4.0;
     const int Y = anchored ? (int)wy0 : 0, X = anchored ? (int)wx0 : 0;
-    double acc[9], accw[9];
+    double acc[9], accw[WEIGHT ? 9 : 1];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      acc[k] = 0.0;
-      accw[k] = 0.0;
-    }
+    for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < (WEIGHT ? 9 : 1); ++k) accw[k] = 0.0;
     // the block's 16 detector values, all loads issued before the first is
-    // used (clamped to the position's first pixel when the ray is off the
-    // detector or past n; `hit` says which rays are real): a load per ray
-    // behind its bounds test waited for each in turn
+    // used (the position's first pixel stands in for a ray off the detector
+    // or past n; `hit` says which rays are real): a load per ray behind its
+    // bounds test waited for each in turn.  One scan position's detector
+    // image is addressed by a 32-bit byte offset from a uniform base (the
+    // host bounds det_h * det_w by 2^30).
+    const char* __restrict__ dimg = (const char*)(data + s * G.det_h * G.det_w);
     float dv[kBpBlock * kBpBlock];
     uint32_t hit = 0;
 #pragma unroll
     for (int r = 0; r < kBpBlock; ++r) {
 #pragma unroll
       for (int c = 0; c < kBpBlock; ++c) {
-        const int iy = min(iy0 + r, n - 1), ix = min(ix0 + c, n - 1);
-        const float tx = th[ix], ty = th[iy];
-        // detector pixel, as in stem_forward_kernel
-        const float xd = fmaf(tx, sh_a, sh_bx), yd = fmaf(ty, sh_a, sh_by);
-        const double dfy = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[0], (double)yd), __dmul_rn(G.det_m[1], (double)xd)),
-                                     G.det_m[2]);
-        const double dfx = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[3], (double)yd), __dmul_rn(G.det_m[4], (double)xd)),
-                                     G.det_m[5]);
+        const double dfy = __dadd_rn(__dadd_rn(ry0[r], cx1[c]), G.det_m[2]);
+        const double dfx = __dadd_rn(__dadd_rn(ry3[r], cx4[c]), G.det_m[5]);
         const int32_t py = tg::xla_f64_to_i32(rint(dfy)), px = tg::xla_f64_to_i32(rint(dfx));
-        const bool ok = iy0 + r < n && ix0 + c < n && py >= 0 && py < G.det_h && px >= 0 && px < G.det_w;
-        dv[r * kBpBlock + c] = data[s * G.det_h * G.det_w + (ok ? (int64_t)py * G.det_w + px : 0)];
+        // (& rather than &&: one straight-line test, no exec-mask branches)
+        const bool ok = (iy0 + r < n) & (ix0 + c < n) & ((uint32_t)py < (uint32_t)G.det_h) &
+                        ((uint32_t)px < (uint32_t)G.det_w);
+        dv[r * kBpBlock + c] = *(const float*)(dimg + (ok ? (uint32_t)(py * G.det_w + px) << 2 : 0u));
         hit |= (uint32_t)ok << (r * kBpBlock + c);
       }
     }
+    __builtin_amdgcn_sched_barrier(0);  // all 16 loads in flight before the deposits
+    // Per ray, straight-line: the window deposit runs for every ray with its
+    // weights and value selected to zero unless the ray is real, on the
+    // image and inside the window.  Rays on the image outside the window
+    // (wide footprints; rare) are marked in `direct` and deposited by the
+    // loop below.  (Conditional blocks in this unrolled loop would let the
+    // compiler sink the gathered loads into them and keep the rays' values
+    // live across them.)
+    uint32_t direct = 0;
 #pragma unroll
     for (int r = 0; r < kBpBlock; ++r) {
-      const int iy = min(iy0 + r, n - 1);
-      const float ty = th[iy];
 #pragma unroll
       for (int c = 0; c < kBpBlock; ++c) {
-        if (!((hit >> (r * kBpBlock + c)) & 1u)) continue;
-        const float tx = th[min(ix0 + c, n - 1)];
-        const double d = (double)dv[r * kBpBlock + c];
+        const bool real = (hit >> (r * kBpBlock + c)) & 1u;
         // bilinear weights at the sample-plane position, as in bilinear_tiled()
-        double fy, fx;
-        sample_pixel(smap, tx, ty, fy, fx);
+        const double fy = fma(smap.ky_y, tyd[r], cy[c]), fx = fma(smap.kx_y, tyd[r], cx[c]);
         const double y0d = floor(fy), x0d = floor(fx);
-        if (!(y0d >= -1.0 && y0d < (double)G.img_h && x0d >= -1.0 && x0d < (double)G.img_w)) continue;
-        const int y0 = (int)y0d, x0 = (int)x0d;
+        const bool on_img = (y0d >= -1.0) & (y0d < (double)G.img_h) & (x0d >= -1.0) & (x0d < (double)G.img_w);
+        // (positions off the image, NaN included, convert a stand-in 0)
+        const int y0 = (int)(on_img ? y0d : 0.0), x0 = (int)(on_img ? x0d : 0.0);
         const float wy = (float)(fy - y0d), wx = (float)(fx - x0d);
         const int dy = y0 - Y, dx = x0 - X;
-        if (anchored && (unsigned)dy <= 1u && (unsigned)dx <= 1u) {
-          // separable weights placed in the 3 x 3 window: row i of the
-          // window carries (1 - wy) at dy and wy at dy + 1, likewise columns
-          const float ry[3] = {dy == 0 ? 1.0f - wy : 0.0f, dy == 0 ? wy : 1.0f - wy, dy == 0 ? 0.0f : wy};
-          const float rx[3] = {dx == 0 ? 1.0f - wx : 0.0f, dx == 0 ? wx : 1.0f - wx, dx == 0 ? 0.0f : wx};
+        const bool win = real & on_img & anchored & ((unsigned)dy <= 1u) & ((unsigned)dx <= 1u);
+        direct |= (uint32_t)(real & on_img & !win) << (r * kBpBlock + c);
+        // separable weights placed in the 3 x 3 window: row i of the window
+        // carries (1 - wy) at dy and wy at dy + 1, likewise columns.  rx * d
+        // is exact in FP64 (two 24-bit significands), so each deposit
+        // ry * rx * d is rounded once, by the FMA into the sum.
+        const float ry[3] = {win & (dy == 0) ? 1.0f - wy : 0.0f, win ? (dy == 0 ? wy : 1.0f - wy) : 0.0f,
+                             win & (dy != 0) ? wy : 0.0f};
+        const float rx[3] = {win & (dx == 0) ? 1.0f - wx : 0.0f, win ? (dx == 0 ? wx : 1.0f - wx) : 0.0f,
+                             win & (dx != 0) ? wx : 0.0f};
+        const double d = (double)(win ? dv[r * kBpBlock + c] : 0.0f);
+        double rxd[3], t[3];
 #pragma unroll
-          for (int i = 0; i < 3; ++i) {
+        for (int k = 0; k < 3; ++k) {
+          rxd[k] = (double)rx[k];
+          t[k] = rxd[k] * d;
+        }
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-              const double w = (double)(ry[i] * rx[k]);
-              acc[3 * i + k] += w * d;
-              accw[3 * i + k] += w;
-            }
+        for (int i = 0; i < 3; ++i) {
+          const double ryd = (double)ry[i];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            acc[3 * i + k] = fma(ryd, t[k], acc[3 * i + k]);
+            if (WEIGHT) accw[3 * i + k] = fma(ryd, rxd[k], accw[3 * i + k]);
           }
-        } else {
+        }
+        // one ray at a time: instruction selection would otherwise defer
+        // every ray's sums past the last ray's set-up and hold all their
+        // weights live at once.  (The empty statements emit nothing; they
+        // make this ray's sums complete before the next ray's begin.)
 #pragma unroll
-          for (int a = 0; a < 2; ++a) {
+        for (int k = 0; k < 9; ++k) {
+          asm volatile("" : "+v"(acc[k]));
+          if (WEIGHT) asm volatile("" : "+v"(accw[k]));
+        }
+      }
+    }
+    // direct deposits: the ray's position and detector value again, by the
+    // same operations, then its four bilinear neighbours straight to the LDS
+    // tile (or global memory outside it)
+    while (direct != 0u) {
+      const int k = __builtin_ctz(direct);
+      direct &= direct - 1u;
+      const int r = k / kBpBlock, c = k % kBpBlock;
+      const float tx = th[ix0 + c], ty = th[iy0 + r];  // real rays lie inside n
+      double fy, fx;
+      sample_pixel(smap, tx, ty, fy, fx);
+      const float xd = fmaf(tx, sh_a, sh_bx), yd = fmaf(ty, sh_a, sh_by);
+      const double dfy = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[0], (double)yd), __dmul_rn(G.det_m[1], (double)xd)),
+                                   G.det_m[2]);
+      const double dfx = __dadd_rn(__dadd_rn(__dmul_rn(G.det_m[3], (double)yd), __dmul_rn(G.det_m[4], (double)xd)),
+                                   G.det_m[5]);
+      const int32_t py = tg::xla_f64_to_i32(rint(dfy)), px = tg::xla_f64_to_i32(rint(dfx));
+      const double dd = (double)*(const float*)(dimg + ((uint32_t)(py * G.det_w + px) << 2));
+      const double y0d = floor(fy), x0d = floor(fx);
+      const int y0 = (int)y0d, x0 = (int)x0d;  // on the image: marked rays only
+      const float wy = (float)(fy - y0d), wx = (float)(fx - x0d);
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-              const double w = (double)((a ? wy : 1.0f - wy) * (b ? wx : 1.0f - wx));
-              const int q = bp_slot(G, oy, ox, y0 + a, x0 + b);
-              if (q >= 0) {
-                atomicAdd(&t_img[q], w * d);
-                if (weight != nullptr) atomicAdd(&t_w[q], w);
-              } else if (q == kBpGlobal) {
-                const int64_t gq = (int64_t)(y0 + a) * G.img_w + (x0 + b);
-                unsafeAtomicAdd(image + gq, w * d);
-                if (weight != nullptr) unsafeAtomicAdd(weight + gq, w);
-              }
-            }
+      for (int a = 0; a < 2; ++a) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          const double w = (double)(a ? wy : 1.0f - wy) * (double)(b ? wx : 1.0f - wx);  // exact
+          const int q = bp_slot(G, oy, ox, y0 + a, x0 + b);
+          if (q >= 0) {
+            atomicAdd(&t_img[q], w * dd);
+            if (WEIGHT) atomicAdd(&t_w[q], w);
+          } else if (q == kBpGlobal) {
+            const int64_t gq = (int64_t)(y0 + a) * G.img_w + (x0 + b);
+            unsafeAtomicAdd(image + gq, w * dd);
+            if (WEIGHT) unsafeAtomicAdd(weight + gq, w);
           }
         }
       }
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      if (accw[k] == 0.0) continue;  // no ray of the block weighted this pixel
+      // untouched window pixels hold exact zeros (as do zero deposits, which
+      // change no sum)
+      if (WEIGHT ? accw[k] == 0.0 : acc[k] == 0.0) continue;
       const int yy = Y + k / 3, xx = X + k % 3;
       const int q = bp_slot(G, oy, ox, yy, xx);
       if (q >= 0) {
         atomicAdd(&t_img[q], acc[k]);
-        if (weight != nullptr) atomicAdd(&t_w[q], accw[k]);
+        if (WEIGHT) atomicAdd(&t_w[q], accw[k]);
       } else if (q == kBpGlobal) {
         const int64_t gq = (int64_t)yy * G.img_w + xx;
         unsafeAtomicAdd(image + gq, acc[k]);
-        if (weight != nullptr) unsafeAtomicAdd(weight + gq, accw[k]);
+        if (WEIGHT) unsafeAtomicAdd(weight + gq, accw[k]);
       }
     }
   }
@@ -435,7 +499,7 @@ __global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_ste
     if (yy < 0 || yy >= G.img_h || xx < 0 || xx >= G.img_w) continue;  // never written
     const int64_t q = (int64_t)yy * G.img_w + xx;
     if (t_img[k] != 0.0) unsafeAtomicAdd(image + q, t_img[k]);
-    if (weight != nullptr && t_w[k] != 0.0) unsafeAtomicAdd(weight + q, t_w[k]);
+    if (WEIGHT && t_w[k] != 0.0) unsafeAtomicAdd(weight + q, t_w[k]);
   }
 }
 
@@ -820,10 +884,16 @@ extern "C" int tg_stem_backproject(const tg_stem_geometry* g, const double* para
   TG_REQUIRE(cam >= 0 && cam < g->n_cam, "tg_stem_backproject: camera %d out of range", cam);
   TG_REQUIRE(g->img_h > 0 && g->img_w > 0 && g->det_h > 0 && g->det_w > 0,
              "tg_stem_backproject: bad sample or detector shape");
+  TG_REQUIRE((int64_t)g->det_h * g->det_w <= (int64_t)1 << 30,
+             "tg_stem_backproject: detector of %lld pixels exceeds 2^30", (long long)g->det_h * g->det_w);
   const int64_t ntile = tg::cdiv((int64_t)g->n_side, (int64_t)kBpRayTile);
   dim3 grid((unsigned)(ntile * ntile), (unsigned)((int64_t)g->scan_h * g->scan_w));
-  hipLaunchKernelGGL(stem_backproject_kernel, grid, dim3(kThreads), 0, (hipStream_t)stream, *g, params, data4d, cam,
-                     image, weight);
+  if (weight != nullptr)
+    hipLaunchKernelGGL(stem_backproject_kernel<true>, grid, dim3(kThreads), 0, (hipStream_t)stream, *g, params, data4d,
+                       cam, image, weight);
+  else
+    hipLaunchKernelGGL(stem_backproject_kernel<false>, grid, dim3(kThreads), 0, (hipStream_t)stream, *g, params,
+                       data4d, cam, image, weight);
   TG_LAUNCH_CHECK("stem_backproject_kernel launch");
   return TG_OK;
 }

@@ -54,11 +54,11 @@ static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // The reference casts rounded pixel coordinates with jnp.astype(int32); XLA's
 // float->int conversion maps NaN to 0 and saturates out-of-range values.
+// Branch-free: clamp into the int32 range (fmax/fmin return the non-NaN
+// operand), truncate, then select 0 for NaN.
 __device__ __forceinline__ int32_t xla_f64_to_i32(double v) {
-  if (v != v) return 0;
-  if (v >= 2147483647.0) return 2147483647;
-  if (v <= -2147483648.0) return (-2147483647 - 1);
-  return (int32_t)v;
+  const int32_t t = (int32_t)fmin(fmax(v, -2147483648.0), 2147483647.0);
+  return v != v ? 0 : t;
 }
 
 }  // namespace tg
