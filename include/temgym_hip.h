@@ -246,7 +246,8 @@ int tg_stem_forward(const tg_stem_geometry* g, const double* params, const float
  * weight[q] += w_q, where w_q are the bilinear weights of sample pixel q at
  * the ray's sample-plane position.  data4d [S, det_h, det_w] f32; image and
  * weight [img_h, img_w] f64, accumulated into (the caller zeroes them).  The
- * FP64 sums are order-dependent only at the last ulp (atomics). */
+ * FP64 sums are order-dependent only at the last ulp (atomics).  One scan
+ * position's detector holds at most 2^30 pixels (TG_EINVAL beyond). */
 int tg_stem_backproject(const tg_stem_geometry* g, const double* params, const float* data4d, int32_t cam,
                         double* image, double* weight, void* stream);
 

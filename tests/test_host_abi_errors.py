@@ -75,6 +75,10 @@ def test_backproject_rejects():
     g = _geometry(n_side=7)
     assert lib.tg_stem_backproject(ctypes.byref(g), BAD, BAD, 0, BAD, None, None) == -1
     assert "n_side" in _err(lib)
+    # one scan position's detector image is addressed by 32-bit byte offsets
+    g = _geometry(det_h=1 << 15, det_w=(1 << 15) + 1)
+    assert lib.tg_stem_backproject(ctypes.byref(g), BAD, BAD, 0, BAD, None, None) == -1
+    assert "exceeds 2^30" in _err(lib)
 
 
 def test_fit_terms_and_fit_reject():
