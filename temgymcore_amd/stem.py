@@ -242,23 +242,23 @@ def stem_forward(model: STEMModel, params: torch.Tensor | None = None, sample: t
     _check_params(model, params)
     S, J, C = model.n_scan, model.n_rays, model.n_cam
     res = {} if out is None else out
-    if coords and "coords" not in res:
-        res["coords"] = torch.empty((C, S, J, 2), dtype=torch.float32, device=device)
-    if values and "values" not in res:
-        res["values"] = torch.empty((S, J), dtype=torch.float32, device=device)
-    if image4d:
-        if "image4d" not in res:
-            res["image4d"] = torch.zeros((S,) + tuple(model.det_shape), dtype=torch.float32, device=device)
+    shapes = {"coords": (C, S, J, 2), "values": (S, J), "image4d": (S,) + tuple(model.det_shape)}
+    for name, wanted in (("coords", coords), ("values", values), ("image4d", image4d)):
+        if not wanted:
+            continue
+        if name not in res:
+            res[name] = torch.empty(shapes[name], dtype=torch.float32, device=device)
         else:
-            # the kernel adds every ray's deposit into the image: a reused
-            # buffer starts from zero on every call (a fill kernel, so the
-            # call stays graph-capturable)
-            img = res["image4d"]
-            if (tuple(img.shape) != (S,) + tuple(model.det_shape) or img.dtype != torch.float32 or not img.is_cuda
-                    or not img.is_contiguous()):
-                raise ValueError(f"out['image4d'] must be a contiguous device float32 tensor of shape "
-                                 f"{(S,) + tuple(model.det_shape)}")
-            img.zero_()
+            # the kernel writes through the raw pointer: a reused buffer must
+            # have exactly the shape and layout it assumes
+            t = res[name]
+            if (not isinstance(t, torch.Tensor) or tuple(t.shape) != shapes[name] or t.dtype != torch.float32
+                    or not t.is_cuda or not t.is_contiguous()):
+                raise ValueError(f"out[{name!r}] must be a contiguous device float32 tensor of shape {shapes[name]}")
+    if image4d:
+        # the kernel adds every ray's deposit into the image: it starts from
+        # zero on every call (a fill kernel, so the call stays graph-capturable)
+        res["image4d"].zero_()
     if (values or image4d) and sample is None:
         raise ValueError("a sample image is needed for values / image4d")
     if sample is not None:

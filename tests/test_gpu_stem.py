@@ -83,6 +83,16 @@ def test_image4d_deposit(device):
     assert again["image4d"] is res["image4d"]
     # (float atomics: the summation order may differ between calls)
     np.testing.assert_allclose(again["image4d"].cpu().numpy().astype(np.float64), ref, rtol=1e-6, atol=1e-5)
+    # reused buffers of the wrong shape, dtype or layout are refused before the launch
+    sample_t = torch.from_numpy(sample).to(device)
+    bad = {
+        "coords": torch.empty((1, model.n_scan, model.n_rays - 1, 2), device=device),
+        "values": torch.empty((model.n_scan, model.n_rays), dtype=torch.float64, device=device),
+        "image4d": torch.empty((model.n_scan,) + tuple(model.det_shape)[::-1] + (2,), device=device)[..., 0],
+    }
+    for name, t in bad.items():
+        with pytest.raises(ValueError, match=f"out\\['{name}'\\]"):
+            stem.stem_forward(model, sample=sample_t, coords=True, values=True, image4d=True, out={name: t})
 
 
 def test_image4d_deposit_full_scan_vs_oracle(device):
