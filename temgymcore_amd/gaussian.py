@@ -347,9 +347,28 @@ def _abcd_per_beamlet(rays: GaussianRay, model, device, n: int):
     return torch.from_numpy(mats[idx.numpy()]).to(device).contiguous(), 25
 
 
+def _check_beamlet_inputs(params: torch.Tensor, mode: int, abcd: torch.Tensor | None, abcd_stride: int) -> None:
+    """The kernels read params [n, 16] and, in propagated mode, n (stride 25)
+    or one (stride 0) 5x5 matrices through raw pointers: refuse anything else."""
+    if not (isinstance(params, torch.Tensor) and params.is_cuda and params.dtype == torch.float64
+            and params.is_contiguous() and params.dim() == 2 and params.shape[1] == NPARAM):
+        raise ValueError(f"beamlet params must be a contiguous device float64 tensor of shape [n, {NPARAM}]")
+    if mode != MODE_PROPAGATED:
+        return
+    n = int(params.shape[0])
+    need = 25 if abcd_stride == 0 else 25 * n
+    if abcd_stride not in (0, 25):
+        raise ValueError(f"abcd_stride must be 0 (one shared matrix) or 25 (one per beamlet), got {abcd_stride}")
+    if not (isinstance(abcd, torch.Tensor) and abcd.is_cuda and abcd.dtype == torch.float64
+            and abcd.is_contiguous() and abcd.device == params.device and abcd.numel() >= need):
+        raise ValueError(f"propagated beamlets need a contiguous device float64 abcd of at least {need} entries "
+                         f"on the params device")
+
+
 def beamlet_coefficients(params: torch.Tensor, grid, mode: int, abcd: torch.Tensor | None = None,
                          abcd_stride: int = 0, cutoff: float | None = None) -> torch.Tensor:
     """Run the f64 coefficient kernel; returns a uint8 tensor of n packed records."""
+    _check_beamlet_inputs(params, mode, abcd, abcd_stride)
     lib = _lib.load()
     n = int(params.shape[0])
     out = torch.empty(n * ctypes.sizeof(BeamletCoef), dtype=torch.uint8, device=params.device)
@@ -400,6 +419,8 @@ def beamlet_backward(params: torch.Tensor, grid, mode: int, grad_field: torch.Te
     or None) for the field cotangent grad_field = dL/dRe + i dL/dIm."""
     lib = _lib.load()
     params = params.detach().contiguous()
+    abcd = None if abcd is None else abcd.detach().contiguous()
+    _check_beamlet_inputs(params, mode, abcd, abcd_stride)
     n = int(params.shape[0])
     H, W = int(grid.shape[0]), int(grid.shape[1])
     g = grad_field.detach().to(device=params.device, dtype=torch.complex64).resolve_conj().contiguous()
@@ -413,7 +434,6 @@ def beamlet_backward(params: torch.Tensor, grid, mode: int, grad_field: torch.Te
     raw = torch.empty(ws_bytes + 256, dtype=torch.uint8, device=params.device)
     off = (-raw.data_ptr()) % 256
     m = _pix2m(grid)
-    abcd = None if abcd is None else abcd.detach().contiguous()
     st = lib.tg_beamlet_backward(params.data_ptr(), n, abcd.data_ptr() if abcd is not None else None, abcd_stride,
                                  mode, m.ctypes.data, float(cutoff or 0.0), H, W,
                                  torch.view_as_real(g).data_ptr(), grad.data_ptr(),

@@ -178,3 +178,14 @@ def test_backward_rejects_bad_arguments(device):
     params = torch.zeros((2, 16), dtype=torch.float64, device=device)
     with pytest.raises(ValueError):
         beamlet_backward(params, grid, MODE_INPUT, torch.zeros((8, 9), dtype=torch.complex64, device=device))
+    # propagated mode: one matrix per beamlet (stride 25) must cover every beamlet
+    one = torch.eye(5, dtype=torch.float64, device=device)[None]
+    cot = torch.zeros((8, 8), dtype=torch.complex64, device=device)
+    with pytest.raises(ValueError, match="at least 50 entries"):
+        beamlet_backward(params, grid, MODE_PROPAGATED, cot, one, 25)
+    with pytest.raises(ValueError, match="at least 50 entries"):
+        beamlet_field(params, grid, MODE_PROPAGATED, one, 25)
+    with pytest.raises(ValueError, match="abcd_stride"):
+        beamlet_field(params, grid, MODE_PROPAGATED, one, 5)
+    with pytest.raises(ValueError, match=r"shape \[n, 16\]"):
+        beamlet_field(params[:, :15], grid, MODE_INPUT)

@@ -235,3 +235,21 @@ def test_gaussian_algebra_helpers_and_map_reduce():
     assert G.map_reduce(lambda p: p[0] * p[1], lambda a, b: a + b, 0.0, pairs) == pytest.approx(11.0)
     d = {"a": np.array([1.0, 2.0]), "b": np.array([5.0, 7.0])}
     assert G.map_reduce(lambda p: p["b"] - p["a"], lambda a, b: a + b, 0.0, d, batch_size=1) == pytest.approx(9.0)
+
+
+def test_beamlet_inputs_are_checked_before_any_launch():
+    """The beamlet kernels read params [n, 16] f64 and the 5x5 matrices through
+    raw pointers: host tensors, other shapes and too few matrices are refused
+    on the host."""
+    import torch
+
+    from temgymcore_amd.components import Detector
+
+    grid = Detector(z=1e-3, pixel_size=(3e-8, 3e-8), shape=(8, 8))
+    with pytest.raises(ValueError, match="device float64"):
+        G.beamlet_coefficients(torch.zeros((2, 16), dtype=torch.float64), grid, G.MODE_INPUT)
+    with pytest.raises(ValueError, match="device float64"):
+        G.beamlet_field(torch.zeros((2, 16), dtype=torch.float32), grid, G.MODE_INPUT)
+    with pytest.raises(ValueError, match="device float64"):
+        G.beamlet_backward(torch.zeros((2, 16), dtype=torch.float64), grid, G.MODE_INPUT,
+                           torch.zeros((8, 8), dtype=torch.complex64))
