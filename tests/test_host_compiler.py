@@ -201,7 +201,10 @@ def test_scalar_api_matches_oracle_and_readme():
 
 def test_run_iter_step_structure():
     # reference test_run.py:10-47: a propagator step before every element, even at d = 0
-    comps = (Scanner(z=1.2, scan_pos_x=0.0, scan_pos_y=0.0), Plane(z=1.2),
+    from temgymcore_amd.propagator import Propagator
+    from temgymcore_amd.source import PointSource
+
+    comps = (PointSource(z=0.0, semi_conv=0.023), Scanner(z=1.2, scan_pos_x=0.0, scan_pos_y=0.0), Plane(z=1.2),
              Descanner(z=1.2, scan_pos_x=0.0, scan_pos_y=0.0), Plane(z=3.1))
     ray = Ray(x=0.12, y=0.23, dx=0.34, dy=0.45, z=3.14, pathlength=0.34)
     res = list(run_iter(ray, comps))
@@ -209,9 +212,37 @@ def test_run_iter_step_structure():
     prev = ray
     for i, c in enumerate(comps):
         prop, pr = res[2 * i]
+        comp, cr = res[2 * i + 1]
+        assert isinstance(prop, Propagator) and isinstance(prop.propagator, FreeSpaceParaxial)
+        assert comp is c
         assert prop.distance == c.z - prev.z
+        assert pr.z == pytest.approx(c.z) and cr.z == pytest.approx(c.z)
+        assert (pr.dx, pr.dy) == (prev.dx, prev.dy)
         assert pr.x == pytest.approx(prev.x + prev.dx * prop.distance)
-        prev = res[2 * i + 1][1]
+        assert pr.y == pytest.approx(prev.y + prev.dy * prop.distance)
+        prev = cr
+
+
+def test_run_iter_without_propagation_composes_the_elements():
+    # reference test_run.py:50-92: every element at one z, so each step is the
+    # element alone and the result is the plain composition
+    from temgymcore_amd.source import PointSource
+
+    z = 1.2
+    comps = (PointSource(z=z, semi_conv=0.023), Scanner(z=z, scan_pos_x=23.0, scan_pos_y=42.0), Plane(z=z),
+             Descanner(z=z, scan_pos_x=13.0, scan_pos_y=11.0), Plane(z=z))
+    ray = Ray(x=0.12, y=0.23, dx=0.34, dy=0.45, z=z, pathlength=0.34)
+    res = list(run_iter(ray, comps))
+    ref = ray
+    for c in comps:
+        ref = c(ref)
+    for i, c in enumerate(comps):
+        assert res[2 * i][0].distance == 0.0
+        assert res[2 * i][1].z == pytest.approx(z) and res[2 * i + 1][1].z == pytest.approx(z)
+    final = res[-1][1]
+    for k in ("x", "y", "dx", "dy", "_one", "z", "pathlength"):
+        assert getattr(final, k) == pytest.approx(getattr(ref, k), rel=1e-15, abs=1e-15), k
+    assert (final.x, final.y) != (ray.x, ray.y)  # the scanner and descanner moved it
 
 
 def test_descanner_jacobian_dual():
