@@ -97,6 +97,51 @@ def test_misaligned_planted_sign_error_is_caught():
     assert np.max(np.abs(np.abs(bad) - np.abs(ref))) > 0.1 * np.abs(ref).max()
 
 
+def _lens_case(tilt, oracle_focal=2e-3):
+    """A shifted, tilted (astigmatic, rotated) Gaussian through drift z1, a thin
+    lens f = 2 mm and drift z2: the oracle's beamlet with the composed ABCD
+    (lens `oracle_focal`) against FFT Fresnel propagation with the lens phase
+    (the transfer-function method of utils.FresnelPropagator, itself pinned by
+    the Gaussian closed form)."""
+    from temgymcore_amd import utils
+
+    lam, z1, fl, z2 = 1e-6, 1e-3, 2e-3, 3e-3
+    k = 2 * np.pi / lam
+    n, side = 1024, 2.048e-3  # 2 um pixels: side * pixel / lam = 4.1 mm >= z1, z2
+    c = (np.arange(n) - n // 2) * (side / n)
+    X, Y = np.meshgrid(c, c)
+    shift = np.array([10e-6, -6e-6])
+    Q1 = B.ray_Q_inv(np.array([[20e-6, 14e-6]]), np.full((1, 2), np.inf), lam, np.array([0.3]))[0]
+    d = np.stack([X - shift[0], Y - shift[1]], -1)
+    E0 = np.exp(1j * (k / 2) * np.einsum("...i,ij,...j->...", d, Q1, d) + 1j * k * (d @ np.asarray(tilt)))
+    E = utils.fresnel_lens_imaging_solution(E0, Y, X, side / n, lam, z1, fl, z2)
+    M = np.array([[1, z2], [0, 1]]) @ np.array([[1, 0], [-1 / oracle_focal, 1]]) @ np.array([[1, z1], [0, 1]])
+    A, Bm, C, D = (M[i, j] * np.eye(2) for i, j in ((0, 0), (0, 1), (1, 0), (1, 1)))
+    Q2 = B.Qinv_ABCD(Q1, A, Bm, C, D)
+    strong = np.abs(E) > 0.05 * np.abs(E).max()
+    idx = np.flatnonzero(strong)[:: max(1, int(strong.sum()) // 300)]
+    pts = np.stack([X.ravel()[idx], Y.ravel()[idx]], -1)
+    z2v = np.zeros(2)
+    got = B.beam_field(1.0, 0.0, Q1, Q2, shift, np.asarray(tilt, float), A, Bm, z2v, z2v, k, pts)
+    return got, E.ravel()[idx]
+
+
+@pytest.mark.parametrize("tilt", [(0.0, 0.0), (3e-3, 2e-3)])
+def test_beamlet_through_a_lens_vs_fresnel_fft(tilt):
+    # the lens case of reference test_gaussians.py:374-517, to a tighter bar
+    got, ref = _lens_case(tilt)
+    assert len(ref) > 100
+    np.testing.assert_allclose(np.abs(got), np.abs(ref), atol=2e-3 * np.abs(ref).max())
+    rel = (got / ref) / (got[0] / ref[0])  # the piston phase is not modelled
+    np.testing.assert_allclose(np.angle(rel), 0.0, atol=5e-3)
+
+
+def test_lens_case_planted_mistake_is_caught():
+    # the oracle with the lens sign flipped against the same Fresnel field
+    got, ref = _lens_case((3e-3, 2e-3), oracle_focal=-2e-3)
+    assert np.max(np.abs(np.abs(got) - np.abs(ref))) > 0.1 * np.abs(ref).max()
+
+
 def test_offset_columns_translate_and_tilt():
     # e translates the field, f multiplies it by exp(i k f.r)
     lam, L = 1e-6, 1e-3
