@@ -298,3 +298,28 @@ def test_separable_input_plane_image_vs_oracle(device):
     got = evaluate_gaussian_input_image(_gray(d), grid).cpu().numpy()
     ref = BO.input_image(r1=BO.grid_coords(grid.pixel_size, grid.shape), **d).reshape(grid.shape)
     _check(got, ref)
+
+
+def _single(x0, y0, dx, dy, lam=500e-9, w0=1e-4):
+    one = lambda v: np.array([v], dtype=np.float64)  # noqa: E731
+    return dict(x=one(x0), y=one(y0), dx=one(dx), dy=one(dy), pathlength=one(0.0), amplitude=one(1.0),
+                waist_xy=np.array([[w0, w0]]), radii=np.full((1, 2), np.inf), wavelength=one(lam), theta=one(0.0))
+
+
+def test_tilted_beam_lands_in_its_quadrant(device):
+    """Reference test_gaussians.py:684-733: a beam tilted by (dx, dy) peaks on
+    the detector 1 m away on the side of its tilt, by dx and dy metres (10
+    pixels here, to within the argmax's pixel)."""
+    det = Detector(z=1.0, pixel_size=(1e-4, 1e-4), shape=(256, 256))
+    xs, ys = det.coords_1d
+    Y, X = np.meshgrid(np.asarray(ys, dtype=np.float64), np.asarray(xs, dtype=np.float64), indexing="ij")
+
+    def peak(img):
+        my, mx = np.unravel_index(np.argmax(np.abs(img)), img.shape)
+        return X[my, mx], Y[my, mx]
+
+    xc, yc = peak(evaluate_gaussian_input_image(_gray(_single(0.0, 0.0, 0.0, 0.0)), det).cpu().numpy())
+    for dx, dy in ((1e-3, 1e-3), (1e-3, -1e-3), (-1e-3, 1e-3), (-1e-3, -1e-3)):
+        xm, ym = peak(make_gaussian_image(_gray(_single(0.0, 0.0, dx, dy)), [det]).cpu().numpy())
+        assert np.sign(xm - xc) == np.sign(dx) and np.sign(ym - yc) == np.sign(dy), (dx, dy, xm, ym)
+        assert abs((xm - xc) - dx) <= 1.5e-4 and abs((ym - yc) - dy) <= 1.5e-4, (dx, dy, xm, ym)
