@@ -408,11 +408,17 @@ __global__ __launch_bounds__(kThreads) void stem_backproject_kernel(const tg_ste
         // carries (1 - wy) at dy and wy at dy + 1, likewise columns.  rx * d
         // is exact in FP64 (two 24-bit significands), so each deposit
         // ry * rx * d is rounded once, by the FMA into the sum.
-        const float ry[3] = {win & (dy == 0) ? 1.0f - wy : 0.0f, win ? (dy == 0 ? wy : 1.0f - wy) : 0.0f,
-                             win & (dy != 0) ? wy : 0.0f};
-        const float rx[3] = {win & (dx == 0) ? 1.0f - wx : 0.0f, win ? (dx == 0 ? wx : 1.0f - wx) : 0.0f,
-                             win & (dx != 0) ? wx : 0.0f};
-        const double d = (double)(win ? dv[r * kBpBlock + c] : 0.0f);
+        float ry[3] = {win & (dy == 0) ? 1.0f - wy : 0.0f, win ? (dy == 0 ? wy : 1.0f - wy) : 0.0f,
+                       win & (dy != 0) ? wy : 0.0f};
+        float rx[3] = {win & (dx == 0) ? 1.0f - wx : 0.0f, win ? (dx == 0 ? wx : 1.0f - wx) : 0.0f,
+                       win & (dx != 0) ? wx : 0.0f};
+        float df = win ? dv[r * kBpBlock + c] : 0.0f;
+        // (the selects stay in f32: without these empty statements the
+        // compiler converts first and selects both halves of each double)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) asm volatile("" : "+v"(ry[k]), "+v"(rx[k]));
+        asm volatile("" : "+v"(df));
+        const double d = (double)df;
         double rxd[3], t[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
