@@ -579,7 +579,7 @@ def bench_backproject(steps: int, warmup: int):
                   "traffic_source": "profiles/stem_backproject_kernel-backproject_64x64x4096.md (calibrated "
                                     "FETCH_SIZE; 4x the algorithmic bytes: one cache line per gathered value)",
                   "note": "gather of one f32 per ray; adjacent rays land four detector pixels apart, so "
-                          "each 64 B sector serves four rays (traffic 4x algorithmic, ~4.5 TB/s of sector "
+                          "each 64 B sector serves four rays (traffic 4x algorithmic, ~4.7 TB/s of sector "
                           "reads), co-limited by the per-ray FP64 position and 3x3 window arithmetic"},
         cpu_baseline=cpu,
     )
