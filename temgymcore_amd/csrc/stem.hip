@@ -265,7 +265,8 @@ __global__ __launch_bounds__(kThreads) void stem_forward_kernel(const tg_stem_ge
 // added to the tile once (9 LDS atomics per 16 rays instead of 64; same-
 // address LDS atomics from the lanes of a wavefront serialise, and bound the
 // one-ray-at-a-time form).  A ray whose cell falls outside the window (wide
-// footprints) deposits directly, as before.
+// footprints) deposits its four neighbours directly to the LDS tile (or to
+// global memory outside it).
 constexpr int kBpTile = 32;
 constexpr int kBpBlock = 4;                        // rays per side of one thread's block
 constexpr int kBpRayTile = 16 * kBpBlock;          // rays per side of one workgroup (16 x 16 blocks)
